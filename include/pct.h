@@ -1,0 +1,256 @@
+/*
+ * pct.h — C-ABI of the MI355X proton-CT reconstruction path.
+ *
+ * The reference (BlakeSchultze/pCT_Reconstruction) has no library boundary: its stages are
+ * `void stage(int num_histories)` functions over global SoA arrays, called in a fixed order
+ * from `preprocessing()` (src/pCT_Reconstruction_Data_Segments_Blake.cu:7394) and
+ * `image_reconstruction()` (same file :10650).  Each entry point below replaces one of those
+ * stage functions; the comment on each names the reference function (file:line, file is
+ * src/pCT_Reconstruction_Data_Segments_Blake.cu unless stated) whose effect it reproduces.
+ *
+ * Conventions
+ *   - every function returns PCT_OK (0) or a negative PCT_E* code; pct_last_error() gives text;
+ *   - host pointers are caller-owned; device memory is owned by the context;
+ *   - one context is used by one host thread at a time; all work runs on the context's stream;
+ *   - units are cm and degrees at the boundary, radians inside (as in the reference).
+ */
+#ifndef PCT_H
+#define PCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCT_OK 0
+#define PCT_E_ARG -1      /* invalid argument / size                        */
+#define PCT_E_STATE -2    /* stage called out of order                      */
+#define PCT_E_HIP -3      /* HIP runtime error                              */
+#define PCT_E_IO -4       /* file I/O or format error                       */
+#define PCT_E_NOMEM -5    /* allocation failure                             */
+#define PCT_E_LIMIT -6    /* size beyond a documented limit                 */
+
+/* Enumerations use the integers of settings.cfg (include/pCT_Reconstruction.h:92-94). */
+enum pct_hull_type { PCT_HULL_IMPORT = 0, PCT_HULL_SC = 1, PCT_HULL_MSC = 2, PCT_HULL_SM = 3, PCT_HULL_FBP = 4 };
+enum pct_filter_type { PCT_FILTER_RAM_LAK = 0, PCT_FILTER_SHEPP_LOGAN = 1, PCT_FILTER_NONE = 2 };
+enum pct_x0_type { PCT_X0_IMPORT = 0, PCT_X0_HULL = 1, PCT_X0_FBP = 2, PCT_X0_HYBRID = 3, PCT_X0_ZEROS = 4 };
+/* DROP per-voxel accumulation precision (the reference uses float atomicAdd, DS.cu:9533). */
+enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1 };
+
+/*
+ * Parameters.  Field names follow the settings.cfg keys (settings.cfg:1-164) and the
+ * `configurations` struct (include/Configurations.h:14-362, defaults at :60-200).
+ * Fields below "derived" are filled by pct_params_derive() with the rules of
+ * set_dependent_parameters (include/Configurations.h:1745-1789).
+ */
+typedef struct pct_params {
+    /* scan geometry and binning */
+    double gantry_angle_interval; /* GANTRY_ANGLE_INTERVAL [deg] */
+    double angular_bin_size;      /* ANGULAR_BIN_SIZE [deg] */
+    double ssd_t_size;            /* SSD_T_SIZE [cm] */
+    double ssd_v_size;            /* SSD_V_SIZE [cm] */
+    double t_shift, u_shift, v_shift; /* T_SHIFT/U_SHIFT/V_SHIFT [cm] */
+    double t_bin_size;            /* T_BIN_SIZE [cm] */
+    double v_bin_size;            /* V_BIN_SIZE [cm] */
+    int32_t sigmas_2_keep;        /* SIGMAS_2_KEEP */
+    double recon_cyl_radius;      /* RECON_CYL_RADIUS [cm] */
+    int32_t columns, rows;        /* COLUMNS, ROWS */
+    double slice_thickness;       /* SLICE_THICKNESS [cm] */
+    /* hull detection */
+    int32_t msc_diff_thresh;      /* MSC_DIFF_THRESH */
+    double sc_lower_threshold, sc_upper_threshold;
+    double msc_lower_threshold, msc_upper_threshold;
+    double sm_lower_threshold, sm_upper_threshold, sm_scale_threshold;
+    int32_t sc_on, msc_on, sm_on, fbp_on; /* SC_ON, MSC_ON, SM_ON, FBP_ON */
+    int32_t hull_type;            /* HULL_TYPE (enum pct_hull_type) */
+    int32_t fbp_filter_type;      /* FBP_FILTER_TYPE (enum pct_filter_type) */
+    int32_t x_0_type;             /* X_0_TYPE (enum pct_x0_type) */
+    double fbp_threshold;         /* FBP_THRESHOLD (DS.h:370) RSP threshold of the FBP hull */
+    /* reconstruction */
+    double lambda;                /* LAMBDA */
+    int32_t iterations;           /* ITERATIONS */
+    int32_t drop_block_size;      /* DROP_BLOCK_SIZE */
+    int32_t histories_per_block;  /* HISTORIES_PER_BLOCK: granularity of the DROP launch (quirk A3) */
+    int32_t min_mlp_length;       /* MIN_MLP_LENGTH (DS.h:409) */
+    /* MLP lookup tables (DS.h:795-807) */
+    double trig_table_min_pis, trig_table_max_pis; /* TRIG_TABLE_MIN/MAX in units of pi */
+    double trig_table_step_degrees;                /* TRIG_TABLE_STEP_DEGREES */
+    double coeff_table_range, coeff_table_step;    /* scattering-coefficient table [cm] */
+    double poly_table_range, poly_table_step;      /* polynomial tables [cm] */
+    /* physics (DS.h:741-745) */
+    double e_0, x_0_rad_length, rsp_air, source_radius;
+    /* behaviour switches; 1 = reproduce the reference */
+    int32_t data_in_mm;           /* DATA_IN_MM: inputs given in mm, converted with *= 0.1 */
+    int32_t quirk_a1_bin0;        /* out-of-range t/v bins keep bin 0 (DS.cu:5380,5431) */
+    int32_t quirk_a3_drop_floor;  /* DROP block processes floor(n/HISTORIES_PER_BLOCK)*HPB (DS.cu:9558) */
+    int32_t quirk_voxel_decode;   /* voxel_2_3D_voxels '>' decode used by FBP (DS.cu:11124-11134) */
+    int32_t drop_accum;           /* enum pct_drop_accum */
+
+    /* ---- derived (pct_params_derive) ---- */
+    int32_t t_bins, v_bins, angular_bins, num_bins;
+    int32_t slices, num_voxels;
+    double recon_cyl_height, image_width, image_height, image_thickness;
+    double voxel_width, voxel_height, voxel_thickness;
+    double x_zero_coordinate, y_zero_coordinate, z_zero_coordinate;
+    double mlp_u_step, ram_lak_tau, pi;
+    double trig_table_min, trig_table_max, trig_table_step; /* radians */
+    int32_t trig_table_elements, coeff_table_elements, poly_table_elements;
+} pct_params;
+
+/* sizeof(pct_params), for bindings that mirror the struct. */
+size_t pct_params_size(void);
+/* Defaults of the configurations constructor (include/Configurations.h:60-200). */
+void pct_params_default(pct_params* p);
+/* Parse a settings.cfg file ("KEY = value // comment", typing of V.cu:9275-9344) over *p. */
+int pct_params_load_cfg(pct_params* p, const char* path, char* err, size_t err_len);
+/* Set one key from text, same typing rules.  Unknown keys return PCT_E_ARG. */
+int pct_params_set(pct_params* p, const char* key, const char* value);
+/* Fill the derived fields (include/Configurations.h:1745-1789); validates sizes. */
+int pct_params_derive(pct_params* p, char* err, size_t err_len);
+
+/* ---------------------------------------------------------------- context ---- */
+typedef struct pct_ctx pct_ctx;
+
+/* Create a context on HIP device `device`; copies *p (which must be derived). */
+int pct_ctx_create(const pct_params* p, int device, pct_ctx** out);
+int pct_ctx_destroy(pct_ctx* ctx);
+const char* pct_last_error(const pct_ctx* ctx);
+/* Block until all queued work of the context has finished. */
+int pct_ctx_synchronize(pct_ctx* ctx);
+/* Number of histories currently held at each stage. */
+enum pct_count_kind { PCT_N_LOADED = 0, PCT_N_RECON_VOLUME = 1, PCT_N_POST_CUTS = 2, PCT_N_RECONSTRUCTION = 3,
+                      PCT_N_LIST_ENTRIES = 4, PCT_N_CURRENT = 5 /* histories held by the latest stage */ };
+int pct_get_count(pct_ctx* ctx, int kind, int64_t* out);
+
+/*
+ * Load raw tracker histories (replaces read_data_chunk_v0, DS.cu:4361-4467, after the file read):
+ * 13 SoA float arrays t0..t3, v0..v3, u0..u3, WEPL in the PCTD order (t_in_1 = t0, t_in_2 = t1,
+ * t_out_1 = t2, t_out_2 = t3), plus the per-history gantry angle in degrees (the integer file
+ * angle for PCTD, the float per-history angle for PCTN).  If params.data_in_mm, coordinates and
+ * WEPL are scaled by 0.1 (convert_mm_2_cm, DS.cu:4070); then the t/u/v shifts are applied
+ * (apply_tuv_shifts, DS.cu:4090).  May be called repeatedly to append.
+ */
+int pct_load_histories(pct_ctx* ctx, size_t n, const float* const t[4], const float* const v[4],
+                       const float* const u[4], const float* wepl, const float* angle_deg);
+
+/* (a)+(b) part 1: recon_volume_intersections (DS.cu:4767, kernel :4890, :5286), binning
+ * (DS.cu:5373, kernel :5482) with the compaction of :5429-5449, and the SC/MSC/SM carving
+ * counts of hull_detection (DS.cu:6607) for the enabled methods. */
+int pct_intersect_bin_carve(pct_ctx* ctx);
+/* (b) part 2: hull_detection_finish (DS.cu:7276): MSC/SM edge detection. */
+int pct_hull_finish(pct_ctx* ctx);
+/* (a) part 2: statistical_calculations_and_cuts (DS.cu:5972) incl. the sinogram sums. */
+int pct_stats_and_cuts(pct_ctx* ctx);
+/* (c): construct_sinogram (DS.cu:6039) + FBP (DS.cu:6080: filter_GPU :6206,
+ * backprojection_GPU2 :6340, FBP_image_2_hull :6592 when HULL_TYPE = FBP). */
+int pct_fbp(pct_ctx* ctx);
+/* hull_selection (DS.cu:7351). */
+int pct_select_hull(pct_ctx* ctx);
+/* define_initial_iterate (DS.cu:8612); x0 is read only for X_0_TYPE = IMPORT (NUM_VOXELS floats). */
+int pct_initial_iterate(pct_ctx* ctx, const float* x0_import);
+/* (d) prerequisite: reconstruction_cuts (DS.cu:8296) with collect_MLP_endpoints_GPU_nobool
+ * (:7819) and the compaction of :8286-8294. */
+int pct_mlp_endpoints(pct_ctx* ctx);
+/* (d): MLP path lists: for every reconstruction history the voxel list of
+ * find_MLP_path_GPU_tabulated (DS.cu:9104-9262) and its effective chord length
+ * (EffectiveChordLength_GPU, :8873).  Built once, reused by every DROP iteration. */
+int pct_mlp_build_lists(pct_ctx* ctx);
+/* (e): `iterations` DROP iterations (DROP_GPU, DS.cu:9848; DROP_full_tx_iteration :9537;
+ * DROP_block_update_GPU :9522; image_update_GPU :9398) on the current image. */
+int pct_drop_iterate(pct_ctx* ctx, int iterations);
+/* Run every stage above in the reference order, with params.iterations DROP iterations. */
+int pct_run_pipeline(pct_ctx* ctx);
+
+/* ---------------------------------------------------------------- outputs ---- */
+enum pct_image_kind { PCT_IMG_X = 0, PCT_IMG_FBP = 1, PCT_IMG_X0 = 2, PCT_IMG_HULL = 3,
+                      PCT_IMG_SC_HULL = 4, PCT_IMG_MSC_COUNTS = 5, PCT_IMG_SM_COUNTS = 6,
+                      PCT_IMG_MSC_HULL = 7, PCT_IMG_SM_HULL = 8, PCT_IMG_FBP_HULL = 9 };
+/* Copy an image (NUM_VOXELS values, index x + y*COLUMNS + z*COLUMNS*ROWS) to host as float. */
+int pct_get_image(pct_ctx* ctx, int kind, float* out);
+/* Overwrite the current iterate x (NUM_VOXELS floats), e.g. a learned warm start. */
+int pct_set_image(pct_ctx* ctx, const float* x);
+/* Hull for HULL_TYPE = IMPORT (NUM_VOXELS bytes, 1 = inside); call before pct_select_hull. */
+int pct_set_hull(pct_ctx* ctx, const uint8_t* hull);
+enum pct_bin_kind { PCT_BIN_COUNTS_PRE = 0, PCT_BIN_MEAN_WEPL = 1, PCT_BIN_MEAN_UT = 2, PCT_BIN_MEAN_UV = 3,
+                    PCT_BIN_STD_WEPL = 4, PCT_BIN_STD_UT = 5, PCT_BIN_STD_UV = 6, PCT_BIN_COUNTS_POST = 7,
+                    PCT_BIN_SINOGRAM = 8, PCT_BIN_SINOGRAM_FILTERED = 9 };
+/* Copy a per-bin array (NUM_BINS values) to host as float (counts converted exactly). */
+int pct_get_bins(pct_ctx* ctx, int kind, float* out);
+/* Per-history arrays of the current stage (count = pct_get_count of that stage). */
+enum pct_hist_kind { PCT_H_X_ENTRY = 0, PCT_H_Y_ENTRY, PCT_H_Z_ENTRY, PCT_H_X_EXIT, PCT_H_Y_EXIT, PCT_H_Z_EXIT,
+                     PCT_H_XY_ENTRY_ANGLE, PCT_H_XZ_ENTRY_ANGLE, PCT_H_XY_EXIT_ANGLE, PCT_H_XZ_EXIT_ANGLE,
+                     PCT_H_WEPL, PCT_H_BIN_NUM /* int32 */, PCT_H_FIRST_VOXEL /* uint32 */,
+                     PCT_H_PATH_COUNT /* uint32 */, PCT_H_CHORD /* float */ };
+int pct_get_histories(pct_ctx* ctx, int kind, void* out, size_t capacity);
+/* MLP list of the reconstruction histories: offsets (n+1 int64) and voxel indices (uint32). */
+int pct_get_lists(pct_ctx* ctx, int64_t* offsets, size_t offsets_cap, uint32_t* voxels, size_t voxels_cap);
+/* Data residual ||A x - b||_2 over the reconstruction histories for the current x, where
+ * row i of A is chord_i on each listed voxel (DROP's system, DS.cu:9163-9174). */
+int pct_residual(pct_ctx* ctx, double* norm);
+/* Stage timings (ms, HIP events) and counts of the last run as a JSON object. */
+const char* pct_metrics_json(pct_ctx* ctx);
+
+/* ---------------------------------------------------------------- DROP kernel boundary ---- */
+/*
+ * One DROP iteration on caller-owned device buffers (no context, no allocation).  This is the hot
+ * loop of DROP_full_tx_iteration (DS.cu:9537-9589) over precomputed path lists:
+ *   offsets[n+1] (int64) into voxels[] (uint32): history i owns voxels[offsets[i] .. offsets[i+1]);
+ *   chord[n] (double, the per-history effective chord length), wepl[n] (float);
+ *   x[num_voxels] (float, updated in place); accum[num_voxels] (double) and counts[num_voxels]
+ *   (uint32) must be zero on entry and are zero on return.
+ * Blocks of block_size histories, each processing floor(m/histories_per_block)*hpb histories
+ * when quirk_a3 != 0.  Work is queued on hip_stream (NULL = default stream); not synchronised.
+ */
+int pct_drop_iteration_device(int64_t n, const int64_t* offsets, const uint32_t* voxels, const double* chord,
+                              const float* wepl, float* x, double* accum, uint32_t* counts, int64_t num_voxels,
+                              int block_size, int histories_per_block, int quirk_a3, int min_mlp_length,
+                              double lambda, void* hip_stream);
+
+/* ---------------------------------------------------------------- I/O ---- */
+/* Read one PCTD/PCTN projection file (count_histories_v0 :3578, read_data_chunk_v0 :4361,
+ * format SupplementalDocumentation/ProposedDataFormat_Version1-1.txt).  With arrays NULL only
+ * the header is read into *n and *angle_deg.  Version 1 / PCTN files carry per-history angles
+ * which are returned in angles (may be NULL). */
+int pct_read_projection(const char* path, int64_t* n, float* header_angle_deg, float* const t[4], float* const v[4],
+                        float* const u[4], float* wepl, float* angles);
+/* Write a PCTD (version 0) or PCTN (version 1, angles != NULL) projection file. */
+int pct_write_projection(const char* path, int64_t n, float angle_deg, float energy_mev, const float* const t[4],
+                         const float* const v[4], const float* const u[4], const float* wepl, const float* angles);
+/* Text image in the array_2_disk layout (DS.cu:10743): COLUMNS values + ' ' per line, ROWS*SLICES lines,
+ * default ostream formatting (6 significant digits). */
+int pct_write_image_text(const char* path, const float* image, int columns, int rows, int slices);
+/* Binary image of write_PNG (DS.cu:9939): float32 value/2 in x-fastest order. */
+int pct_write_image_dat(const char* path, const float* image, int columns, int rows, int slices);
+
+/* ---------------------------------------------------------------- synthetic data ---- */
+/* Phantom made of up to 16 objects; each is a cylinder (kind 0, axis z) or an ellipsoid (kind 1)
+ * with centre, semi-axes (cylinder: radius a, half height c), rotation about z [deg] and an RSP
+ * that is ADDED where the object lies (Shepp-Logan convention). */
+typedef struct pct_object { int32_t kind; double cx, cy, cz, a, b, c, phi_deg, rsp; } pct_object;
+typedef struct pct_phantom { int32_t n_objects; pct_object obj[16]; } pct_phantom;
+typedef struct pct_beam {
+    double energy_mev;             /* kinetic energy */
+    double u_planes[4];            /* tracker u positions [cm], e.g. -20,-15,15,20 */
+    double t_half_range, v_half_range; /* entry position ranges [cm] at u_planes[1] */
+    int32_t continuous_angles;     /* 0: n_angles equally spaced integer angles; 1: U[0,360) */
+    int32_t n_angles; double angle_interval;
+    double wepl_noise_sigma;       /* [cm] */
+    double outlier_fraction;       /* fraction of histories with WEPL replaced by U(0, 2*max) */
+    int32_t scatter;               /* 1: Highland multiple-scattering deviation of the exit */
+} pct_beam;
+void pct_phantom_cylinder_c1(pct_phantom* ph);   /* water cylinder + 3 inserts */
+void pct_phantom_head(pct_phantom* ph);          /* 10-ellipsoid head */
+void pct_phantom_random(pct_phantom* ph, uint64_t seed, int n_ellipsoids, double radius, double half_height);
+/* Generate n histories (cm, angles in degrees) sorted by projection (angle-major) as in the
+ * per-angle projection files; deterministic in (seed, n) for any thread count. */
+int pct_generate_histories(const pct_phantom* ph, const pct_beam* beam, uint64_t seed, int64_t n, float* const t[4],
+                           float* const v[4], float* const u[4], float* wepl, float* angle_deg);
+/* Rasterise the phantom's RSP on the reconstruction grid of *p (voxel centres, NUM_VOXELS floats). */
+int pct_phantom_rasterize(const pct_phantom* ph, const pct_params* p, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCT_H */

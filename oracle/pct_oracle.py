@@ -1,0 +1,334 @@
+"""Python binding of the CPU oracle (oracle/_build/libpct_oracle.so) and its pipeline driver.
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Imported by tests/, by
+__graft_entry__.smoke() and by the cpu_baseline leg of bench.py; the product never imports it.
+
+`run_pipeline` strings the oracle's stage functions together in the reference's order
+(preprocessing DS.cu:7394 -> image_reconstruction DS.cu:10650), doing the host compactions of
+the reference (DS.cu:5429-5449, 5892-5911, 8286-8294) with numpy.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_LIB = Path(__file__).resolve().parent / "_build" / "libpct_oracle.so"
+_L = None
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is not None:
+        return _L
+    if not ORACLE_LIB.exists():
+        raise RuntimeError(f"{ORACLE_LIB} missing: run pct_reconstruction_amd/build.py")
+    L = C.CDLL(str(ORACLE_LIB))
+    vp, i64, P = C.c_void_p, C.c_int64, C.POINTER
+    f3 = C.c_void_p * 3
+    f4 = C.c_void_p * 4
+    f10 = C.c_void_p * 10
+    from pct_reconstruction_amd._lib import Params
+    sig = {
+        "o_derive": ([P(Params)], None),
+        "o_intersections": ([P(Params), i64, f4, f4, f4, vp, f10, vp], None),
+        "o_binning": ([P(Params), i64, f10, vp, vp], None),
+        "o_stats_and_cuts": ([P(Params), i64, vp, vp, vp, f10, vp, f3, f3, vp, vp, vp], None),
+        "o_construct_sinogram": ([P(Params), vp, vp], None),
+        "o_init_hull_u8": ([P(Params), vp], None),
+        "o_init_hull_i32": ([P(Params), vp], None),
+        "o_sc": ([P(Params), i64, f10, vp, vp], None),
+        "o_msc": ([P(Params), i64, f10, vp, vp], None),
+        "o_sm": ([P(Params), i64, f10, vp, vp], None),
+        "o_msc_edge": ([P(Params), vp, vp], None),
+        "o_sm_edge": ([P(Params), vp, vp], None),
+        "o_filter": ([P(Params), vp, vp], None),
+        "o_backproject": ([P(Params), vp, vp], None),
+        "o_fbp_hull": ([P(Params), vp, vp], None),
+        "o_tables_create": ([P(Params), vp], C.c_int),
+        "o_tables_destroy": ([vp], None),
+        "o_endpoints": ([P(Params), vp, i64, f10, vp], None),
+        "o_chord_length": ([P(Params), C.c_double, C.c_double], C.c_double),
+        "o_mlp_lists": ([P(Params), vp, i64, f10, vp, vp, vp, i64, vp], i64),
+        "o_drop_iteration": ([P(Params), i64, vp, vp, vp, vp, vp, C.c_int, C.c_int], None),
+        "o_drop_iteration_fp64": ([P(Params), i64, vp, vp, vp, vp, vp], None),
+        "o_residual": ([i64, vp, vp, vp, vp, vp, C.c_int], C.c_double),
+    }
+    for name, (a, r) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = a
+        fn.restype = r
+    _L = L
+    return L
+
+
+class Tables(C.Structure):
+    _fields_ = [("n_trig", C.c_int64), ("n_coeff", C.c_int64), ("n_poly", C.c_int64)] + [
+        (k, C.POINTER(C.c_double)) for k in ("sin_t", "cos_t", "scat", "p12", "p23", "p34", "p26", "p312")]
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def _arr(arrs, n):
+    return (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+
+
+H_NAMES = ["x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "xy_entry_angle", "xz_entry_angle",
+           "xy_exit_angle", "xz_exit_angle"]
+
+
+def derive(P):
+    q = P.copy()
+    lib().o_derive(C.byref(q))
+    return q
+
+
+def intersections(P, t, v, u, angle):
+    n = angle.shape[0]
+    h = np.zeros((10, n), np.float32)
+    missed = np.zeros(n, np.uint8)
+    tt = [np.ascontiguousarray(x, np.float32) for x in t]
+    vv = [np.ascontiguousarray(x, np.float32) for x in v]
+    uu = [np.ascontiguousarray(x, np.float32) for x in u]
+    ang = np.ascontiguousarray(angle, np.float32)
+    lib().o_intersections(C.byref(P), n, _arr(tt, 4), _arr(vv, 4), _arr(uu, 4), _p(ang), _arr(list(h), 10), _p(missed))
+    return h, missed.astype(bool)
+
+
+def binning(P, h):
+    n = h.shape[1]
+    b = np.zeros(n, np.int32)
+    r = np.zeros(n, np.uint8)
+    lib().o_binning(C.byref(P), n, _arr(list(h), 10), _p(b), _p(r))
+    return b, r
+
+
+def stats_and_cuts(P, bin_num, in_range, wepl, h):
+    n = wepl.shape[0]
+    NB = P.num_bins
+    counts_pre = np.zeros(NB, np.int32)
+    mean = np.zeros((3, NB), np.float32)
+    std = np.zeros((3, NB), np.float32)
+    counts_post = np.zeros(NB, np.int32)
+    sino = np.zeros(NB, np.float32)
+    failed = np.zeros(n, np.uint8)
+    lib().o_stats_and_cuts(C.byref(P), n, _p(bin_num), _p(in_range), _p(wepl), _arr(list(h), 10), _p(counts_pre),
+                           _arr(list(mean), 3), _arr(list(std), 3), _p(counts_post), _p(sino), _p(failed))
+    return dict(counts_pre=counts_pre, mean=mean, std=std, counts_post=counts_post, sinogram_sum=sino,
+                failed=failed.astype(bool))
+
+
+def init_hull(P, dtype):
+    out = np.zeros(P.num_voxels, dtype)
+    if dtype == np.uint8:
+        lib().o_init_hull_u8(C.byref(P), _p(out))
+    else:
+        lib().o_init_hull_i32(C.byref(P), _p(out))
+    return out
+
+
+def carve(P, method, h, wepl):
+    n = wepl.shape[0]
+    if method == "sc":
+        out = init_hull(P, np.uint8)
+        lib().o_sc(C.byref(P), n, _arr(list(h), 10), _p(wepl), _p(out))
+    else:
+        out = init_hull(P, np.int32)
+        getattr(lib(), "o_" + method)(C.byref(P), n, _arr(list(h), 10), _p(wepl), _p(out))
+    return out
+
+
+def msc_edge(P, counts):
+    out = np.zeros(P.num_voxels, np.int32)
+    lib().o_msc_edge(C.byref(P), _p(np.ascontiguousarray(counts, np.int32)), _p(out))
+    return out
+
+
+def sm_edge(P, counts):
+    out = np.zeros(P.num_voxels, np.int32)
+    lib().o_sm_edge(C.byref(P), _p(np.ascontiguousarray(counts, np.int32)), _p(out))
+    return out
+
+
+def construct_sinogram(P, counts_post, sino_sum):
+    s = sino_sum.copy()
+    lib().o_construct_sinogram(C.byref(P), _p(counts_post), _p(s))
+    return s
+
+
+def filter_sinogram(P, sino):
+    out = np.zeros(P.num_bins, np.float32)
+    lib().o_filter(C.byref(P), _p(np.ascontiguousarray(sino, np.float32)), _p(out))
+    return out
+
+
+def backproject(P, filtered):
+    out = np.zeros(P.num_voxels, np.float32)
+    lib().o_backproject(C.byref(P), _p(np.ascontiguousarray(filtered, np.float32)), _p(out))
+    return out
+
+
+def fbp_hull(P, image):
+    out = np.zeros(P.num_voxels, np.uint8)
+    lib().o_fbp_hull(C.byref(P), _p(np.ascontiguousarray(image, np.float32)), _p(out))
+    return out
+
+
+def endpoints(P, hull_u8, h):
+    h = np.ascontiguousarray(h.copy(), np.float32)
+    n = h.shape[1]
+    first = np.zeros(n, np.uint32)
+    lib().o_endpoints(C.byref(P), _p(np.ascontiguousarray(hull_u8, np.uint8)), n, _arr(list(h), 10), _p(first))
+    return h, first
+
+
+def chord_length(P, angle_t, angle_v):
+    return lib().o_chord_length(C.byref(P), float(angle_t), float(angle_v))
+
+
+class MlpTables:
+    def __init__(self, P):
+        self.t = Tables()
+        if lib().o_tables_create(C.byref(P), C.byref(self.t)) != 0:
+            raise MemoryError("oracle tables")
+
+    def __del__(self):
+        try:
+            lib().o_tables_destroy(C.byref(self.t))
+        except Exception:
+            pass
+
+    def array(self, name):
+        n = {"sin_t": self.t.n_trig, "cos_t": self.t.n_trig, "scat": self.t.n_coeff}.get(name, self.t.n_poly)
+        return np.ctypeslib.as_array(getattr(self.t, name), shape=(n,)).copy()
+
+
+def mlp_lists(P, tables: MlpTables, h, first):
+    h = np.ascontiguousarray(h, np.float32)
+    n = h.shape[1]
+    off = np.zeros(n + 1, np.int64)
+    chord = np.zeros(n, np.float64)
+    cap = max(16, int(n) * (int(2 * P.recon_cyl_radius / P.mlp_u_step) + 4))
+    vox = np.zeros(cap, np.uint32)
+    total = lib().o_mlp_lists(C.byref(P), C.byref(tables.t), n, _arr(list(h), 10),
+                              _p(np.ascontiguousarray(first, np.uint32)), _p(off), _p(vox), cap, _p(chord))
+    if total < 0:
+        raise RuntimeError("oracle MLP capacity")
+    return off, vox[:total].copy(), chord
+
+
+def drop_iteration(P, off, vox, chord, wepl, x, accum=1, nthreads=1):
+    x = np.ascontiguousarray(x, np.float32).copy()
+    lib().o_drop_iteration(C.byref(P), off.shape[0] - 1, _p(off), _p(vox), _p(chord),
+                           _p(np.ascontiguousarray(wepl, np.float32)), _p(x), int(accum), int(nthreads))
+    return x
+
+
+def drop_iteration_fp64(P, off, vox, chord, wepl, x64):
+    x = np.ascontiguousarray(x64, np.float64).copy()
+    lib().o_drop_iteration_fp64(C.byref(P), off.shape[0] - 1, _p(off), _p(vox), _p(chord),
+                                _p(np.ascontiguousarray(wepl, np.float32)), _p(x))
+    return x
+
+
+def residual(off, vox, chord, wepl, x, min_len):
+    return lib().o_residual(off.shape[0] - 1, _p(off), _p(vox), _p(chord), _p(np.ascontiguousarray(wepl, np.float32)),
+                            _p(np.ascontiguousarray(x, np.float32).reshape(-1)), int(min_len))
+
+
+def convert_units(P, t, v, u, wepl):
+    """convert_mm_2_cm (DS.cu:4070) and apply_tuv_shifts (DS.cu:4090) in float as the reference does."""
+    def mm(a):
+        return (a.astype(np.float64) * 0.1).astype(np.float32) if P.data_in_mm else a
+
+    def sh(a, s):
+        return (a.astype(np.float64) + s).astype(np.float32) if (P.t_shift or P.u_shift or P.v_shift) else a
+    t = sh(mm(np.asarray(t, np.float32)), P.t_shift)
+    u = sh(mm(np.asarray(u, np.float32)), P.u_shift)
+    v = sh(mm(np.asarray(v, np.float32)), P.v_shift)
+    return t, v, u, mm(np.asarray(wepl, np.float32))
+
+
+def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=False):
+    """The reference pipeline on the CPU.  Returns a dict with every intermediate result."""
+    P = derive(P)
+    out = {}
+    t, v, u, w = convert_units(P, data["t"], data["v"], data["u"], data["wepl"])
+    h, missed = intersections(P, t, v, u, data["angle_deg"])
+    bin_num, in_range = binning(P, h)
+    keep = ~missed
+    h = np.ascontiguousarray(h[:, keep])
+    w = np.ascontiguousarray(w[keep])
+    bin_num = np.ascontiguousarray(bin_num[keep])
+    in_range = np.ascontiguousarray(in_range[keep])
+    out.update(h_volume=h, wepl_volume=w, bin_num=bin_num, in_range=in_range, missed=missed)
+    if P.sc_on:
+        out["sc_hull"] = carve(P, "sc", h, w)
+    if P.msc_on:
+        out["msc_counts"] = carve(P, "msc", h, w)
+        out["msc_hull"] = msc_edge(P, out["msc_counts"])
+    if P.sm_on:
+        out["sm_counts"] = carve(P, "sm", h, w)
+        out["sm_hull"] = sm_edge(P, out["sm_counts"])
+    if stop_after == "hulls":
+        return out
+    st = stats_and_cuts(P, bin_num, in_range, w, h)
+    out.update(st)
+    keep = ~st["failed"]
+    h = np.ascontiguousarray(h[:, keep])
+    w = np.ascontiguousarray(w[keep])
+    out.update(h_cut=h, wepl_cut=w)
+    if stop_after == "cuts":
+        return out
+    sino = construct_sinogram(P, st["counts_post"], st["sinogram_sum"])
+    out["sinogram"] = sino
+    if P.fbp_on:
+        out["sinogram_filtered"] = filter_sinogram(P, sino)
+        out["fbp"] = backproject(P, out["sinogram_filtered"])
+        if P.hull_type == 4:
+            out["fbp_hull"] = fbp_hull(P, out["fbp"])
+    hull = {1: lambda: out["sc_hull"], 2: lambda: (out["msc_hull"] == 1).astype(np.uint8),
+            3: lambda: (out["sm_hull"] == 1).astype(np.uint8), 4: lambda: out["fbp_hull"]}[P.hull_type]()
+    hull = np.ascontiguousarray(hull, np.uint8)
+    out["hull"] = hull
+    if P.x_0_type == 1:
+        x0 = hull.astype(np.float32)
+    elif P.x_0_type == 2:
+        x0 = out["fbp"].copy()
+    elif P.x_0_type == 3:
+        x0 = np.where(hull != 0, out["fbp"], np.float32(0)).astype(np.float32)
+    else:
+        x0 = np.zeros(P.num_voxels, np.float32)
+    out["x0"] = x0
+    if stop_after == "fbp":
+        return out
+    h2, first = endpoints(P, hull, h)
+    keep = first != P.num_voxels
+    h2 = np.ascontiguousarray(h2[:, keep])
+    w2 = np.ascontiguousarray(w[keep])
+    first = np.ascontiguousarray(first[keep])
+    out.update(h_recon=h2, wepl_recon=w2, first_voxel=first)
+    if stop_after == "endpoints":
+        return out
+    tabs = MlpTables(P)
+    off, vox, chord = mlp_lists(P, tabs, h2, first)
+    out.update(offsets=off, voxels=vox, chord=chord)
+    if stop_after == "lists":
+        return out
+    its = P.iterations if iterations is None else iterations
+    x = x0.copy()
+    x64 = x0.astype(np.float64)
+    xs = []
+    for _ in range(its):
+        x = drop_iteration(P, off, vox, chord, w2, x, accum=accum)
+        if with_fp64:
+            x64 = drop_iteration_fp64(P, off, vox, chord, w2, x64)
+        xs.append(x.copy())
+    out.update(x=x, x_iterations=xs)
+    if with_fp64:
+        out["x_fp64"] = x64
+    return out

@@ -1,0 +1,182 @@
+// internal.h — context layout and launcher declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pct.h"
+#include "devmath.h"
+
+#define PCT_CHECK_HIP(ctx, expr)                                                        \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+            return PCT_E_HIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+#define PCT_TRY(expr)               \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != PCT_OK) return rc_; \
+    } while (0)
+
+enum pct_stage {
+    ST_EMPTY = 0,
+    ST_LOADED,
+    ST_BINNED,      // intersections + binning + carving counts done
+    ST_HULLS,       // edge detection done
+    ST_CUT,         // statistical cuts done
+    ST_FBP,         // sinogram + FBP done
+    ST_HULL_SEL,    // hull selected
+    ST_X0,          // initial iterate defined
+    ST_ENDPOINTS,   // MLP endpoints + compaction
+    ST_LISTS,       // MLP lists built
+};
+
+// Device copies of the MLP lookup tables (DS.cu:8421-8502) plus the per-step Sigma_1 terms.
+struct DevTables {
+    double* sin_t = nullptr;
+    double* cos_t = nullptr;
+    double* scat = nullptr;
+    double* p12 = nullptr;
+    double* p26 = nullptr;
+    double* p312 = nullptr;
+    double* step_terms = nullptr;  // [k][8]: u1_12, u1_23, p34(u1), S1I0, S1I1, S1I2, unused, unused
+    int n_steps = 0;               // rows of step_terms
+    int64_t n_trig = 0, n_coeff = 0, n_poly = 0;
+};
+
+struct pct_ctx {
+    pct_params P;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::string metrics;
+    int stage = ST_EMPTY;
+
+    // raw tracker data (kept so the pipeline can be re-run): t0..t3, v0..v3, u0..u3, wepl, angle
+    int64_t n_loaded = 0, cap_loaded = 0;
+    float* raw[14] = {};
+
+    // per-history arrays of the current stage
+    int64_t n_hist = 0, cap_hist = 0;
+    int64_t n_recon_volume = 0, n_post_cuts = 0, n_recon = 0;
+    float* h[10] = {};        // x/y/z entry, x/y/z exit, xy/xz entry angle, xy/xz exit angle
+    float* wepl = nullptr;
+    int32_t* bin_num = nullptr;
+    uint8_t* in_range = nullptr;
+    uint32_t* first_voxel = nullptr;
+    // scratch of the same capacity
+    float* h_tmp[10] = {};
+    float* wepl_tmp = nullptr;
+    int32_t* bin_tmp = nullptr;
+    uint8_t* u8_tmp = nullptr;
+    uint8_t* flag = nullptr;
+    int64_t* pos = nullptr;
+    uint32_t* u32_a = nullptr;
+    uint32_t* u32_b = nullptr;
+    uint32_t* u32_c = nullptr;
+    uint32_t* u32_d = nullptr;
+    void* cub_tmp = nullptr;
+    size_t cub_tmp_bytes = 0;
+
+    // bins
+    int32_t* counts_pre = nullptr;
+    int32_t* counts_post = nullptr;
+    int32_t* bin_off = nullptr;  // NB + 1
+    float* mean[3] = {};
+    float* stddev[3] = {};
+    float* sinogram = nullptr;
+    float* sino_filtered = nullptr;
+
+    // images
+    uint32_t* sc_carved = nullptr;  // bit-packed carved mask for SC
+    uint8_t* sc_hull = nullptr;
+    int32_t* msc_counts = nullptr;
+    int32_t* sm_counts = nullptr;
+    int32_t* msc_hull = nullptr;
+    int32_t* sm_hull = nullptr;
+    uint8_t* fbp_hull = nullptr;
+    uint8_t* hull = nullptr;
+    uint8_t* hull_import = nullptr;
+    float* fbp = nullptr;
+    float* x0 = nullptr;
+    float* x = nullptr;
+
+    // MLP path lists (d): CSR with capacity per history; entries [off, off + cnt)
+    int64_t* list_off = nullptr;   // n_recon + 1 (capacity offsets)
+    uint32_t* list_cnt = nullptr;  // n_recon
+    uint32_t* list_vox = nullptr;
+    double* chord = nullptr;       // n_recon
+    int64_t list_cap = 0, list_entries = 0;
+    DevTables T;
+
+    // DROP scratch
+    double* accum64 = nullptr;
+    float* accum32 = nullptr;
+    uint32_t* S = nullptr;
+    uint32_t* touched = nullptr;
+    uint32_t* n_touched = nullptr;  // 2 counters (ping-pong)
+    int32_t* list_overflow = nullptr;
+
+    // timing (ms) of the last run of each stage
+    double t_stage[16] = {};
+};
+
+// allocation helpers
+template <class T>
+int pct_alloc(pct_ctx* ctx, T** p, size_t count)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e);
+        *p = nullptr;
+        return PCT_E_NOMEM;
+    }
+    return PCT_OK;
+}
+template <class T>
+void pct_free(T** p)
+{
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+}
+
+DevGeom pct_geom(const pct_params& P);
+
+// host-side tables (tables.cpp)
+struct HostTables {
+    std::vector<double> sin_t, cos_t, scat, p12, p23, p34, p26, p312;
+};
+void pct_build_tables(const pct_params& P, HostTables* t);
+// per-step Sigma_1 terms for steps k = 0..n-1 (u_1 = (k+1) * MLP_U_STEP)
+void pct_build_step_terms(const pct_params& P, const HostTables& t, int n, std::vector<double>* out);
+
+// stage launchers (one per translation unit)
+int pct_stage_intersect_bin_carve(pct_ctx* ctx);
+int pct_stage_hull_finish(pct_ctx* ctx);
+int pct_stage_stats_and_cuts(pct_ctx* ctx);
+int pct_stage_fbp(pct_ctx* ctx);
+int pct_stage_select_hull(pct_ctx* ctx);
+int pct_stage_initial_iterate(pct_ctx* ctx, const float* x0_import);
+int pct_stage_endpoints(pct_ctx* ctx);
+int pct_stage_build_lists(pct_ctx* ctx);
+int pct_stage_drop(pct_ctx* ctx, int iterations);
+int pct_stage_residual(pct_ctx* ctx, double* norm);
+
+// compaction: keep entries with flag != 0 (flag array has ctx->n_hist entries); returns new count
+int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bool with_first_voxel);
+// DROP iteration on raw device buffers
+int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* cnt, const uint32_t* vox,
+                 const double* chord, const float* wepl, float* x, double* accum64, float* accum32, uint32_t* S,
+                 uint32_t* touched, uint32_t* n_touched, int64_t num_voxels, int block_size, int hpb, int quirk_a3,
+                 int min_len, double lambda, int accum_mode, std::string* err);

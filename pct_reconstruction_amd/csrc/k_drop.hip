@@ -1,0 +1,268 @@
+// k_drop.hip — stage (e): DROP block-iterative updates over precomputed MLP lists.
+//
+// Per DROP block (DROP_full_tx_iteration, DS.cu:9537-9589):
+//   k_drop_scatter  one wavefront per history: 64 lanes read 64 consecutive list entries
+//                   (coalesced), gather x, reduce a_i.x in FP64 across the wavefront, then every
+//                   lane adds the history's update to its voxels' sums (calculate_x_update_GPU :9321
+//                   + DROP_block_update_GPU :9522).  The per-voxel numerator is accumulated in FP64
+//                   (PCT_ACCUM_FP64: 53-bit sums, order effects ~1e-16 relative) or in FP32 as the
+//                   reference does; the denominator S is an exact u32 count.  The first lane to
+//                   raise S[v] from 0 appends v to a touched-voxel list (wavefront-aggregated
+//                   append: one atomic per wavefront and step).
+//   k_drop_apply    x[v] += sum[v]/S[v] for the touched voxels only, and clears sum/S
+//                   (image_update_GPU :9398 does this densely over every voxel).
+// The touched list makes the apply cost proportional to the block's footprint instead of NUM_VOXELS.
+#include "internal.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <bool FP64, bool TOUCHED>
+__global__ void __launch_bounds__(kBlock) k_drop_scatter(int64_t start, int64_t proc, const int64_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ cnt_arr,
+                                                          const uint32_t* __restrict__ vox,
+                                                          const double* __restrict__ chord,
+                                                          const float* __restrict__ wepl, const float* __restrict__ x,
+                                                          double* __restrict__ acc64, float* __restrict__ acc32,
+                                                          uint32_t* __restrict__ S, uint32_t* __restrict__ touched,
+                                                          uint32_t* __restrict__ n_touched, int slot, int min_len,
+                                                          double lambda)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    if (TOUCHED && wave == 0 && lane == 0) n_touched[slot ^ 1] = 0;  // consumed by the previous block's apply
+    for (int64_t p = start + wave; p < start + proc; p += n_waves) {
+        const int64_t base = off[p];
+        const int64_t cnt = cnt_arr ? (int64_t)cnt_arr[p] : off[p + 1] - base;
+        const double L = chord[p];
+        double part = 0.0;
+        for (int64_t j = lane; j < cnt; j += kWave) part += (double)x[vox[base + j]];
+        double ax = wave_sum(part) * L;
+        double aa = (double)pct_sqf(L) * (double)cnt;
+        if (cnt < min_len) continue;  // uniform across the wavefront
+        const double val = L * (((double)wepl[p] - ax) / aa) * lambda;
+        for (int64_t j0 = 0; j0 < cnt; j0 += kWave) {
+            const int64_t j = j0 + lane;
+            bool fresh = false;
+            uint32_t v = 0;
+            if (j < cnt) {
+                v = vox[base + j];
+                uint32_t old = atomicAdd(&S[v], 1u);
+                fresh = (old == 0);
+                if (FP64) atomicAdd(&acc64[v], val);
+                else atomicAdd(&acc32[v], (float)val);
+            }
+            if (TOUCHED) {
+                uint64_t m = __ballot(fresh);
+                if (m) {
+                    int leader = __ffsll((unsigned long long)m) - 1;
+                    uint32_t b = 0;
+                    if (lane == leader) b = atomicAdd(&n_touched[slot], (uint32_t)__popcll(m));
+                    b = __shfl(b, leader, kWave);
+                    if (fresh) touched[b + __popcll(m & ((1ull << lane) - 1ull))] = v;
+                }
+            }
+        }
+    }
+}
+
+template <bool FP64>
+__global__ void k_drop_apply_touched(const uint32_t* __restrict__ touched, const uint32_t* __restrict__ n_touched,
+                                     int slot, float* __restrict__ x, double* __restrict__ acc64,
+                                     float* __restrict__ acc32, uint32_t* __restrict__ S)
+{
+    const uint32_t n = n_touched[slot];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t v = touched[t];
+        const uint32_t s = S[v];
+        if (FP64) {
+            x[v] += (float)(acc64[v] / (double)s);
+            acc64[v] = 0.0;
+        } else {
+            x[v] += acc32[v] / (float)s;
+            acc32[v] = 0.0f;
+        }
+        S[v] = 0;
+    }
+}
+
+template <bool FP64>
+__global__ void k_drop_apply_dense(int64_t nv, float* __restrict__ x, double* __restrict__ acc64,
+                                   float* __restrict__ acc32, uint32_t* __restrict__ S)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = S[v];
+        if (s > 0) {
+            if (FP64) {
+                x[v] += (float)(acc64[v] / (double)s);
+                acc64[v] = 0.0;
+            } else {
+                x[v] += acc32[v] / (float)s;
+                acc32[v] = 0.0f;
+            }
+            S[v] = 0;
+        }
+    }
+}
+
+}  // namespace
+
+int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* cnt, const uint32_t* vox,
+                 const double* chord, const float* wepl, float* x, double* acc64, float* acc32, uint32_t* S,
+                 uint32_t* touched, uint32_t* n_touched, int64_t num_voxels, int block_size, int hpb, int quirk_a3,
+                 int min_len, double lambda, int accum_mode, std::string* err)
+{
+    const bool fp64 = accum_mode == PCT_ACCUM_FP64;
+    const bool use_touched = touched != nullptr && n_touched != nullptr;
+    if (use_touched) {
+        hipError_t e = hipMemsetAsync(n_touched, 0, 2 * sizeof(uint32_t), s);
+        if (e != hipSuccess) {
+            *err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
+            return PCT_E_HIP;
+        }
+    }
+    int k = 0;
+    for (int64_t start = 0; start < n; start += block_size, k++) {
+        int64_t m = n - start < block_size ? n - start : block_size;
+        int64_t proc = quirk_a3 ? (m / hpb) * hpb : m;  // DS.cu:9558: integer division floors
+        if (proc <= 0) continue;
+        int slot = k & 1;
+        int64_t waves = proc;
+        int64_t blocks = (waves * kWave + kBlock - 1) / kBlock;
+        if (blocks > 8192) blocks = 8192;
+        dim3 grid((unsigned)blocks), blk(kBlock);
+        if (fp64) {
+            if (use_touched)
+                hipLaunchKernelGGL((k_drop_scatter<true, true>), grid, blk, 0, s, start, proc, off, cnt, vox, chord,
+                                   wepl, x, acc64, acc32, S, touched, n_touched, slot, min_len, lambda);
+            else
+                hipLaunchKernelGGL((k_drop_scatter<true, false>), grid, blk, 0, s, start, proc, off, cnt, vox, chord,
+                                   wepl, x, acc64, acc32, S, touched, n_touched, slot, min_len, lambda);
+        } else {
+            if (use_touched)
+                hipLaunchKernelGGL((k_drop_scatter<false, true>), grid, blk, 0, s, start, proc, off, cnt, vox, chord,
+                                   wepl, x, acc64, acc32, S, touched, n_touched, slot, min_len, lambda);
+            else
+                hipLaunchKernelGGL((k_drop_scatter<false, false>), grid, blk, 0, s, start, proc, off, cnt, vox, chord,
+                                   wepl, x, acc64, acc32, S, touched, n_touched, slot, min_len, lambda);
+        }
+        if (use_touched) {
+            int64_t ab = (proc * 64 + kBlock - 1) / kBlock;  // a block touches at most sum(cnt) voxels
+            if (ab > 2048) ab = 2048;
+            if (fp64)
+                hipLaunchKernelGGL(k_drop_apply_touched<true>, dim3((unsigned)ab), dim3(kBlock), 0, s, touched, n_touched,
+                                   slot, x, acc64, acc32, S);
+            else
+                hipLaunchKernelGGL(k_drop_apply_touched<false>, dim3((unsigned)ab), dim3(kBlock), 0, s, touched,
+                                   n_touched, slot, x, acc64, acc32, S);
+        } else {
+            int64_t ab = (num_voxels + kBlock - 1) / kBlock;
+            if (ab > 4096) ab = 4096;
+            if (fp64)
+                hipLaunchKernelGGL(k_drop_apply_dense<true>, dim3((unsigned)ab), dim3(kBlock), 0, s, num_voxels, x,
+                                   acc64, acc32, S);
+            else
+                hipLaunchKernelGGL(k_drop_apply_dense<false>, dim3((unsigned)ab), dim3(kBlock), 0, s, num_voxels, x,
+                                   acc64, acc32, S);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        *err = std::string("DROP launch: ") + hipGetErrorString(e);
+        return PCT_E_HIP;
+    }
+    return PCT_OK;
+}
+
+int pct_stage_drop(pct_ctx* ctx, int iterations)
+{
+    const pct_params& P = ctx->P;
+    const int64_t NV = P.num_voxels;
+    if (!ctx->S) {
+        PCT_TRY(pct_alloc(ctx, &ctx->S, NV));
+        PCT_TRY(pct_alloc(ctx, &ctx->touched, NV));
+        PCT_TRY(pct_alloc(ctx, &ctx->n_touched, 2));
+        PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->S, 0, NV * sizeof(uint32_t), ctx->stream));
+    }
+    if (P.drop_accum == PCT_ACCUM_FP64 && !ctx->accum64) {
+        PCT_TRY(pct_alloc(ctx, &ctx->accum64, NV));
+        PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->accum64, 0, NV * sizeof(double), ctx->stream));
+    }
+    if (P.drop_accum == PCT_ACCUM_FP32 && !ctx->accum32) {
+        PCT_TRY(pct_alloc(ctx, &ctx->accum32, NV));
+        PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->accum32, 0, NV * sizeof(float), ctx->stream));
+    }
+    for (int it = 0; it < iterations; it++) {
+        int rc = pct_drop_run(ctx->stream, ctx->n_recon, ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord,
+                              ctx->wepl, ctx->x, ctx->accum64, ctx->accum32, ctx->S, ctx->touched, ctx->n_touched, NV,
+                              P.drop_block_size, P.histories_per_block, P.quirk_a3_drop_floor, P.min_mlp_length,
+                              P.lambda, P.drop_accum, &ctx->err);
+        if (rc != PCT_OK) return rc;
+    }
+    return PCT_OK;
+}
+
+namespace {
+__global__ void k_residual_rows(int64_t n, const int64_t* __restrict__ off, const uint32_t* __restrict__ cnt_arr,
+                                const uint32_t* __restrict__ vox, const double* __restrict__ chord,
+                                const float* __restrict__ wepl, const float* __restrict__ x, int min_len,
+                                double* __restrict__ r2)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    for (int64_t p = wave; p < n; p += n_waves) {
+        const int64_t base = off[p];
+        const int64_t cnt = cnt_arr[p];
+        double part = 0.0;
+        for (int64_t j = lane; j < cnt; j += kWave) part += (double)x[vox[base + j]];
+        double s = wave_sum(part);
+        if (lane == 0) {
+            double r = (double)wepl[p] - chord[p] * s;
+            r2[p] = cnt >= min_len ? r * r : 0.0;
+        }
+    }
+}
+}  // namespace
+
+#include <hipcub/hipcub.hpp>
+int pct_stage_residual(pct_ctx* ctx, double* norm)
+{
+    const int64_t n = ctx->n_recon;
+    hipStream_t s = ctx->stream;
+    double* r2 = nullptr;
+    double* out = nullptr;
+    PCT_TRY(pct_alloc(ctx, &r2, n));
+    PCT_TRY(pct_alloc(ctx, &out, 1));
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(out, 0, sizeof(double), s));
+    if (n > 0) {
+        int64_t blocks = (n * kWave + kBlock - 1) / kBlock;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)blocks), dim3(kBlock), 0, s, n, ctx->list_off, ctx->list_cnt,
+                           ctx->list_vox, ctx->chord, ctx->wepl, ctx->x, ctx->P.min_mlp_length, r2);
+        size_t need = 0;
+        PCT_CHECK_HIP(ctx, hipcub::DeviceReduce::Sum(nullptr, need, r2, out, n, s));
+        if (need > ctx->cub_tmp_bytes) {
+            PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
+            ctx->cub_tmp_bytes = need;
+        }
+        PCT_CHECK_HIP(ctx, hipcub::DeviceReduce::Sum(ctx->cub_tmp, need, r2, out, n, s));
+    }
+    double h = 0.0;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    pct_free(&r2);
+    pct_free(&out);
+    *norm = std::sqrt(h);
+    return PCT_OK;
+}

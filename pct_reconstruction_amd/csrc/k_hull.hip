@@ -1,0 +1,384 @@
+// k_hull.hip — stage (b): SC / MSC / SM hull carving, edge detection, FBP hull and hull selection.
+//
+// Carving is one lane per history walking its straight entry->exit chord voxel by voxel (SC_GPU
+// DS.cu:6681, MSC_GPU :6799, SM_GPU :6997).  SC clears voxels: here the walk sets bits in a
+// bit-packed "carved" mask with 32-bit atomicOr (one word per 32 voxels, 1/8 of the bytes of the
+// reference's bool image); the hull is the cylinder mask without the carved bits.  MSC/SM count
+// traversals with 32-bit integer atomics, which are exact and order independent.
+#include "internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock)
+{
+    int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// initialize_hull_GPU, DS.cu:6659-6669
+__device__ __forceinline__ int inside_cylinder(const DevGeom& g, int column, int row)
+{
+    double x = (column - g.C / 2 + 0.5) * g.VW;
+    double y = (g.R / 2 - row - 0.5) * g.VH;
+    return (pct_sqf(x) + pct_sqf(y)) < pct_sqf(g.Rc);
+}
+
+__global__ void k_init_counts(DevGeom g, int32_t* counts)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.NV) return;
+    int c = (int)(v % g.C), r = (int)((v / g.C) % g.R);
+    counts[v] = inside_cylinder(g, c, r);
+}
+
+// The entry->exit walk of SC/MSC/SM (take_2D_step_GPU :11346, take_3D_step_GPU :11392).  Visits
+// every voxel from the entry voxel up to, excluding, the exit voxel.  The walk also stops at a
+// negative voxel coordinate or after 4*(C+R+S)+16 steps: there the reference indexes out of bounds
+// or does not terminate.
+template <class Visit>
+__device__ void carve_walk(const DevGeom& g, float xe, float ye, float ze, float xx, float yx, float zx, Visit visit)
+{
+    const int C = g.C, R = g.R, S = g.S;
+    double x = xe, y = ye, z = ze;
+    int x_move = (xe <= xx) - (xe >= xx);
+    int y_move = (ye <= yx) - (ye >= yx);
+    int z_move = (ze <= zx) - (ze >= zx);
+    int vx = pct_calc_voxel(g.X0, xe, g.VW), vy = pct_calc_voxel(g.Y0, ye, g.VH), vz = pct_calc_voxel(g.Z0, ze, g.VT);
+    int64_t voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
+    double x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+    double y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+    double z_to_go = pct_dist(g.Z0, z, PCT_Z_INC, z_move, g.VT, vz);
+    double dy_dx = (double)((yx - ye) / (xx - xe));
+    double dz_dx = (double)((zx - ze) / (xx - xe));
+    double dz_dy = (double)((zx - ze) / (yx - ye));
+    double dx_dy = (double)((xx - xe) / (yx - ye));
+    double dx_dz = (double)((xx - xe) / (zx - ze));
+    double dy_dz = (double)((yx - ye) / (zx - ze));
+    int vxo = pct_calc_voxel(g.X0, xx, g.VW), vyo = pct_calc_voxel(g.Y0, yx, g.VH), vzo = pct_calc_voxel(g.Z0, zx, g.VT);
+    int64_t voxel_out = vxo + (int64_t)vyo * C + (int64_t)vzo * C * R;
+    bool end_walk = (voxel == voxel_out) || (vx >= C) || (vy >= R) || (vz >= S) || vx < 0 || vy < 0 || vz < 0;
+    if (!end_walk) visit(voxel);
+    const int cap = 4 * (C + R + S) + 16;
+    int steps = 0;
+    while (!end_walk) {
+        if (z_move != 0) {
+            double x_ext = fabs(dz_dx) * x_to_go;
+            double y_ext = fabs(dz_dy) * y_to_go;
+            if ((z_to_go <= x_ext) && (z_to_go <= y_ext)) {
+                vz -= z_move;
+                z = pct_edge(g.Z0, vz, g.VT, PCT_Z_INC, z_move);
+                x = pct_corr(dx_dz, z, ze, xe);
+                y = pct_corr(dy_dz, z, ze, ye);
+                x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+                y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+                z_to_go = g.VT;
+            } else if (x_ext <= y_ext) {
+                vx += x_move;
+                x = pct_edge(g.X0, vx, g.VW, PCT_X_INC, x_move);
+                y = pct_corr(dy_dx, x, xe, ye);
+                z = pct_corr(dz_dx, x, xe, ze);
+                x_to_go = g.VW;
+                y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+                z_to_go = pct_dist(g.Z0, z, PCT_Z_INC, z_move, g.VT, vz);
+            } else {
+                vy -= y_move;
+                y = pct_edge(g.Y0, vy, g.VH, PCT_Y_INC, y_move);
+                x = pct_corr(dx_dy, y, ye, xe);
+                z = pct_corr(dz_dy, y, ye, ze);
+                x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+                y_to_go = g.VH;
+                z_to_go = pct_dist(g.Z0, z, PCT_Z_INC, z_move, g.VT, vz);
+            }
+            if (x_to_go == 0) { x_to_go = g.VW; vx += x_move; }
+            if (y_to_go == 0) { y_to_go = g.VH; vy -= y_move; }
+            if (z_to_go == 0) { z_to_go = g.VT; vz -= z_move; }
+        } else {
+            double y_ext = fabs(dx_dy) * y_to_go;
+            if (x_to_go <= y_ext) {
+                vx += x_move;
+                x = pct_edge(g.X0, vx, g.VW, PCT_X_INC, x_move);
+                y = pct_corr(dy_dx, x, xe, ye);
+                x_to_go = g.VW;
+                y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+            } else {
+                vy -= y_move;
+                y = pct_edge(g.Y0, vy, g.VH, PCT_Y_INC, y_move);
+                x = pct_corr(dx_dy, y, ye, xe);
+                x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+                y_to_go = g.VH;
+            }
+            if (x_to_go == 0) { x_to_go = g.VW; vx += x_move; }
+            if (y_to_go == 0) { y_to_go = g.VH; vy -= y_move; }
+        }
+        vz = vz > 0 ? vz : 0;
+        voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
+        end_walk = (voxel == voxel_out) || (vx >= C) || (vy >= R) || (vz >= S);
+        if (vx < 0 || vy < 0 || ++steps > cap) end_walk = true;
+        if (!end_walk) visit(voxel);
+    }
+}
+
+struct HistPtrs {
+    const float *xe, *ye, *ze, *xx, *yx, *zx, *wepl;
+};
+
+// method: 0 SC (WEPL in [SC_LOWER, SC_UPPER], DS.cu:6686), 1 MSC ([MSC_LOWER, MSC_UPPER], :6806),
+// 2 SM (WEPL >= SM_LOWER, :7004)
+__global__ void k_carve(int64_t n, DevGeom g, HistPtrs hp, int method, double lo, double hi, uint32_t* carved,
+                        int32_t* counts)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float w = hp.wepl[i];
+    bool take = (method == 2) ? (w >= lo) : ((w <= hi) && (w >= lo));
+    if (!take) return;
+    if (method == 0) {
+        carve_walk(g, hp.xe[i], hp.ye[i], hp.ze[i], hp.xx[i], hp.yx[i], hp.zx[i],
+                   [&](int64_t v) { atomicOr(&carved[v >> 5], 1u << (v & 31)); });
+    } else {
+        carve_walk(g, hp.xe[i], hp.ye[i], hp.ze[i], hp.xx[i], hp.yx[i], hp.zx[i],
+                   [&](int64_t v) { atomicAdd(&counts[v], 1); });
+    }
+}
+
+// SC hull = cylinder mask without the carved voxels
+__global__ void k_sc_finish(DevGeom g, const uint32_t* carved, uint8_t* hull)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.NV) return;
+    int c = (int)(v % g.C), r = (int)((v / g.C) % g.R);
+    bool cut = (carved[v >> 5] >> (v & 31)) & 1u;
+    hull[v] = (uint8_t)(inside_cylinder(g, c, r) && !cut);
+}
+
+__device__ __forceinline__ bool on_rim(const DevGeom& g, int column, int row)
+{
+    float x = (float)((column - g.C / 2 + 0.5) * g.VW);
+    float y = (float)((g.R / 2 - row - 0.5) * g.VH);
+    double m = g.VW > g.VH ? g.VW : g.VH;
+    return (x * x + y * y) >= pct_sqf(g.Rc - m / 2);
+}
+
+__device__ __forceinline__ int max_diff(const DevGeom& g, const int32_t* counts, int c, int r, int64_t so)
+{
+    int md = 0;
+    if ((r != 0) && (r != g.R - 1) && (c != 0) && (c != g.C - 1)) {
+        int32_t own = counts[so + c + r * g.C];
+        for (int rr = r - 1; rr <= r + 1; rr++)
+            for (int cc = c - 1; cc <= c + 1; cc++) {
+                int d = own - counts[so + cc + rr * g.C];
+                if (d > md) md = d;
+            }
+    }
+    return md;
+}
+
+// MSC_edge_detection_GPU (DS.cu:6911-6957) reading unmodified counts (the race-free semantics of the
+// two-buffer version :6958-6985).
+__global__ void k_msc_edge(DevGeom g, const int32_t* counts, int32_t* hull, int thresh)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.NV) return;
+    int c = (int)(v % g.C), r = (int)((v / g.C) % g.R);
+    int64_t so = (v / ((int64_t)g.C * g.R)) * g.C * g.R;
+    int md = max_diff(g, counts, c, r, so);
+    int32_t out = (md > thresh || counts[v] > thresh) ? 0 : 1;
+    if (on_rim(g, c, r)) out = 0;
+    hull[v] = out;
+}
+
+// carve_differences (DS.cu:6616) + the host threshold scan (:7118-7134): per slice, the count at the
+// first pixel (in pixel order) holding the largest 3x3 difference.  One workgroup per slice; ties
+// resolved to the lowest pixel index.
+__global__ void k_sm_threshold(DevGeom g, const int32_t* counts, int32_t* thr)
+{
+    __shared__ int s_diff[kBlock];
+    __shared__ int s_pix[kBlock];
+    const int s = blockIdx.x;
+    const int64_t so = (int64_t)s * g.C * g.R;
+    int best = 0, best_pix = 0x7fffffff;
+    for (int pix = threadIdx.x; pix < g.C * g.R; pix += blockDim.x) {
+        int d = max_diff(g, counts, pix % g.C, pix / g.C, so);
+        if (d > best || (d == best && d > 0 && pix < best_pix)) {
+            best = d;
+            best_pix = pix;
+        }
+    }
+    s_diff[threadIdx.x] = best;
+    s_pix[threadIdx.x] = best_pix;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            int d2 = s_diff[threadIdx.x + w], p2 = s_pix[threadIdx.x + w];
+            int d1 = s_diff[threadIdx.x], p1 = s_pix[threadIdx.x];
+            if (d2 > d1 || (d2 == d1 && p2 < p1)) {
+                s_diff[threadIdx.x] = d2;
+                s_pix[threadIdx.x] = p2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) thr[s] = (s_diff[0] > 0) ? counts[so + s_pix[0]] : 0;
+}
+
+// SM_edge_detection_GPU, DS.cu:7160-7175
+__global__ void k_sm_edge(DevGeom g, const int32_t* counts, const int32_t* thr, int32_t* hull, double scale)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.NV) return;
+    int c = (int)(v % g.C), r = (int)((v / g.C) % g.R);
+    int s = (int)(v / ((int64_t)g.C * g.R));
+    int32_t out = (counts[v] > scale * thr[s]) ? 1 : 0;
+    if (on_rim(g, c, r)) out = 0;
+    hull[v] = out;
+}
+
+// FBP_image_2_hull_GPU, DS.cu:6592-6603
+__global__ void k_fbp_hull(DevGeom g, const float* image, uint8_t* hull, double threshold)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.NV) return;
+    int c = (int)(v % g.C), r = (int)((v / g.C) % g.R);
+    double x = -g.Rc + (c + 0.5) * g.VW;
+    double y = g.Rc - (r + 0.5) * g.VH;
+    double d2 = (double)(pct_sqf(x) + pct_sqf(y));
+    hull[v] = (uint8_t)(((double)image[v] > threshold) && (d2 < (double)pct_sqf(g.Rc)));
+}
+
+__global__ void k_int_to_bool(int64_t n, const int32_t* in, uint8_t* out)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) out[v] = (uint8_t)(in[v] == 1);
+}
+
+}  // namespace
+
+DevGeom pct_geom(const pct_params& P)
+{
+    DevGeom g;
+    g.C = P.columns;
+    g.R = P.rows;
+    g.S = P.slices;
+    g.NV = P.num_voxels;
+    g.X0 = P.x_zero_coordinate;
+    g.Y0 = P.y_zero_coordinate;
+    g.Z0 = P.z_zero_coordinate;
+    g.VW = P.voxel_width;
+    g.VH = P.voxel_height;
+    g.VT = P.voxel_thickness;
+    g.Rc = P.recon_cyl_radius;
+    g.H = P.recon_cyl_height;
+    g.PI = P.pi;
+    return g;
+}
+
+// hull_initializations (DS.cu:6636) + the carving counts of hull_detection (:6607)
+int pct_carve_counts(pct_ctx* ctx)
+{
+    const pct_params& P = ctx->P;
+    hipStream_t s = ctx->stream;
+    DevGeom g = pct_geom(P);
+    const int64_t NV = P.num_voxels, n = ctx->n_hist;
+    HistPtrs hp{ctx->h[0], ctx->h[1], ctx->h[2], ctx->h[3], ctx->h[4], ctx->h[5], ctx->wepl};
+    if (P.sc_on) {
+        PCT_TRY(pct_alloc(ctx, &ctx->sc_carved, (NV + 31) / 32));
+        PCT_TRY(pct_alloc(ctx, &ctx->sc_hull, NV));
+        PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->sc_carved, 0, ((NV + 31) / 32) * sizeof(uint32_t), s));
+        if (n > 0)
+            hipLaunchKernelGGL(k_carve, dim3(grid_for(n)), dim3(kBlock), 0, s, n, g, hp, 0, P.sc_lower_threshold,
+                               P.sc_upper_threshold, ctx->sc_carved, (int32_t*)nullptr);
+        hipLaunchKernelGGL(k_sc_finish, dim3(grid_for(NV)), dim3(kBlock), 0, s, g, ctx->sc_carved, ctx->sc_hull);
+    }
+    if (P.msc_on) {
+        PCT_TRY(pct_alloc(ctx, &ctx->msc_counts, NV));
+        hipLaunchKernelGGL(k_init_counts, dim3(grid_for(NV)), dim3(kBlock), 0, s, g, ctx->msc_counts);
+        if (n > 0)
+            hipLaunchKernelGGL(k_carve, dim3(grid_for(n)), dim3(kBlock), 0, s, n, g, hp, 1, P.msc_lower_threshold,
+                               P.msc_upper_threshold, (uint32_t*)nullptr, ctx->msc_counts);
+    }
+    if (P.sm_on) {
+        PCT_TRY(pct_alloc(ctx, &ctx->sm_counts, NV));
+        hipLaunchKernelGGL(k_init_counts, dim3(grid_for(NV)), dim3(kBlock), 0, s, g, ctx->sm_counts);
+        if (n > 0)
+            hipLaunchKernelGGL(k_carve, dim3(grid_for(n)), dim3(kBlock), 0, s, n, g, hp, 2, P.sm_lower_threshold,
+                               P.sm_upper_threshold, (uint32_t*)nullptr, ctx->sm_counts);
+    }
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+// hull_detection_finish, DS.cu:7276-7340
+int pct_stage_hull_finish(pct_ctx* ctx)
+{
+    const pct_params& P = ctx->P;
+    hipStream_t s = ctx->stream;
+    DevGeom g = pct_geom(P);
+    const int64_t NV = P.num_voxels;
+    if (P.msc_on) {
+        PCT_TRY(pct_alloc(ctx, &ctx->msc_hull, NV));
+        hipLaunchKernelGGL(k_msc_edge, dim3(grid_for(NV)), dim3(kBlock), 0, s, g, ctx->msc_counts, ctx->msc_hull,
+                           P.msc_diff_thresh);
+    }
+    if (P.sm_on) {
+        PCT_TRY(pct_alloc(ctx, &ctx->sm_hull, NV));
+        int32_t* thr = nullptr;
+        PCT_TRY(pct_alloc(ctx, &thr, P.slices));
+        hipLaunchKernelGGL(k_sm_threshold, dim3(P.slices), dim3(kBlock), 0, s, g, ctx->sm_counts, thr);
+        hipLaunchKernelGGL(k_sm_edge, dim3(grid_for(NV)), dim3(kBlock), 0, s, g, ctx->sm_counts, thr, ctx->sm_hull,
+                           P.sm_scale_threshold);
+        PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+        pct_free(&thr);
+    }
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    ctx->stage = ST_HULLS;
+    return PCT_OK;
+}
+
+int pct_fbp_hull(pct_ctx* ctx)
+{
+    const pct_params& P = ctx->P;
+    DevGeom g = pct_geom(P);
+    PCT_TRY(pct_alloc(ctx, &ctx->fbp_hull, P.num_voxels));
+    hipLaunchKernelGGL(k_fbp_hull, dim3(grid_for(P.num_voxels)), dim3(kBlock), 0, ctx->stream, g, ctx->fbp,
+                       ctx->fbp_hull, P.fbp_threshold);
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+// hull_selection, DS.cu:7351-7393
+int pct_stage_select_hull(pct_ctx* ctx)
+{
+    const pct_params& P = ctx->P;
+    hipStream_t s = ctx->stream;
+    const int64_t NV = P.num_voxels;
+    PCT_TRY(pct_alloc(ctx, &ctx->hull, NV));
+    switch (P.hull_type) {
+    case PCT_HULL_SC:
+        if (!ctx->sc_hull) { ctx->err = "HULL_TYPE = SC requires SC_ON"; return PCT_E_STATE; }
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->hull, ctx->sc_hull, NV, hipMemcpyDeviceToDevice, s));
+        break;
+    case PCT_HULL_MSC:
+        if (!ctx->msc_hull) { ctx->err = "HULL_TYPE = MSC requires MSC_ON"; return PCT_E_STATE; }
+        hipLaunchKernelGGL(k_int_to_bool, dim3(grid_for(NV)), dim3(kBlock), 0, s, NV, ctx->msc_hull, ctx->hull);
+        break;
+    case PCT_HULL_SM:
+        if (!ctx->sm_hull) { ctx->err = "HULL_TYPE = SM requires SM_ON"; return PCT_E_STATE; }
+        hipLaunchKernelGGL(k_int_to_bool, dim3(grid_for(NV)), dim3(kBlock), 0, s, NV, ctx->sm_hull, ctx->hull);
+        break;
+    case PCT_HULL_FBP:
+        if (!ctx->fbp_hull) { ctx->err = "HULL_TYPE = FBP requires FBP_ON"; return PCT_E_STATE; }
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->hull, ctx->fbp_hull, NV, hipMemcpyDeviceToDevice, s));
+        break;
+    case PCT_HULL_IMPORT:
+        if (!ctx->hull_import) { ctx->err = "HULL_TYPE = IMPORT requires a hull set with pct_set_hull"; return PCT_E_STATE; }
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->hull, ctx->hull_import, NV, hipMemcpyDeviceToDevice, s));
+        break;
+    default:
+        ctx->err = "invalid HULL_TYPE";
+        return PCT_E_ARG;
+    }
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    ctx->stage = ST_HULL_SEL;
+    return PCT_OK;
+}

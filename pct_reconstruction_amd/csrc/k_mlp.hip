@@ -1,0 +1,457 @@
+// k_mlp.hip — stage (d): MLP endpoints on the hull and the MLP voxel lists.
+//
+// Endpoints: one lane per history, two straight walks along the tracker angles to the first hull
+// voxel (find_MLP_endpoints_GPU, DS.cu:7581-7773).
+//
+// MLP lists: a persistent one-history-per-lane kernel.  The Sigma_1 terms of every u-step (they
+// depend only on the step number) are precomputed on the host in FP64 and staged in LDS at kernel
+// start; per step a lane reads them from LDS, one scattering coefficient for its Sigma_2 from the
+// FP64 table in HBM/L2, and does the 2x2 inversions and the t/v solve in FP64 exactly as
+// find_MLP_path_GPU_tabulated (DS.cu:9104-9262).  Each history's voxel list is written to its own
+// segment of a CSR array whose capacity comes from the number of u-steps; the DROP kernels read
+// entries [off, off + count).
+#include <hipcub/hipcub.hpp>
+
+#include "internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock)
+{
+    int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// find_MLP_endpoints_GPU, DS.cu:7581-7773; the walk is bounded by 4*(C+R+S)+16 steps (the
+// reference loops forever when the walk never leaves the image, e.g. z clamped at slice 0).
+__device__ int find_endpoint(const DevGeom& g, const uint8_t* __restrict__ hull, double x_start, double y_start,
+                             double z_start, double xy_angle, double xz_angle, double* xo, double* yo, double* zo,
+                             int* vxo, int* vyo, int* vzo, int entering)
+{
+    const int C = g.C, R = g.R, S = g.S;
+    double x = x_start, y = y_start, z = z_start;
+    if (!entering) xy_angle += g.PI;
+    const double cxy = cos(xy_angle), sxy = sin(xy_angle), sxz = sin(xz_angle);
+    int x_move = (cxy >= 0) - (cxy <= 0);
+    int y_move = (sxy >= 0) - (sxy <= 0);
+    int z_move = (sxz >= 0) - (sxz <= 0);
+    if (x_move < 0) z_move *= -1;
+    int vx = pct_calc_voxel(g.X0, x, g.VW), vy = pct_calc_voxel(g.Y0, y, g.VH), vz = pct_calc_voxel(g.Z0, z, g.VT);
+    double x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+    double y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+    double z_to_go = pct_dist(g.Z0, z, PCT_Z_INC, z_move, g.VT, vz);
+    int64_t voxel = vx + ((int64_t)vy + (int64_t)vz * R) * C;
+    const double txy = tan(xy_angle), txz = tan(xz_angle);
+    double delta_yx = fabs(txy);
+    double delta_zx = fabs(txz);
+    double delta_zy = fabs(txz / txy);
+    double dy_dx = txy;
+    double dz_dx = txz;
+    double dz_dy = txz / txy;
+    double dx_dy = (double)pct_recipf(txy);
+    double dx_dz = (double)pct_recipf(txz);
+    double dy_dz = (double)pct_tanf(xy_angle) / txz;
+    int hit = 0;
+    bool outside = (vx >= C) || (vy >= R) || (vz >= S) || (vx < 0) || (vy < 0) || (vz < 0);
+    if (!outside) hit = (hull[voxel] == 1);
+    bool end_walk = outside || hit;
+    const int cap = 4 * (C + R + S) + 16;
+    int steps = 0;
+    while (!end_walk) {
+        if (z_move != 0) {
+            double x_ext = delta_zx * x_to_go;
+            double y_ext = delta_zy * y_to_go;
+            if ((z_to_go <= x_ext) && (z_to_go <= y_ext)) {
+                vz -= z_move;
+                z = pct_edge(g.Z0, vz, g.VT, PCT_Z_INC, z_move);
+                x = pct_corr(dx_dz, z, z_start, x_start);
+                y = pct_corr(dy_dz, z, z_start, y_start);
+                x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+                y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+                z_to_go = g.VT;
+            } else if (x_ext <= y_ext) {
+                vx += x_move;
+                x = pct_edge(g.X0, vx, g.VW, PCT_X_INC, x_move);
+                y = pct_corr(dy_dx, x, x_start, y_start);
+                z = pct_corr(dz_dx, x, x_start, z_start);
+                x_to_go = g.VW;
+                y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+                z_to_go = pct_dist(g.Z0, z, PCT_Z_INC, z_move, g.VT, vz);
+            } else {
+                vy -= y_move;
+                y = pct_edge(g.Y0, vy, g.VH, PCT_Y_INC, y_move);
+                x = pct_corr(dx_dy, y, y_start, x_start);
+                z = pct_corr(dz_dy, y, y_start, z_start);
+                x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+                y_to_go = g.VH;
+                z_to_go = pct_dist(g.Z0, z, PCT_Z_INC, z_move, g.VT, vz);
+            }
+            if (x_to_go == 0) { x_to_go = g.VW; vx += x_move; }
+            if (y_to_go == 0) { y_to_go = g.VH; vy -= y_move; }
+            if (z_to_go == 0) { z_to_go = g.VT; vz -= z_move; }
+            vz = vz > 0 ? vz : 0;
+        } else {
+            double y_ext = y_to_go / delta_yx;
+            if (x_to_go <= y_ext) {
+                vx += x_move;
+                x = pct_edge(g.X0, vx, g.VW, PCT_X_INC, x_move);
+                y = pct_corr(dy_dx, x, x_start, y_start);
+                x_to_go = g.VW;
+                y_to_go = pct_dist(g.Y0, y, PCT_Y_INC, y_move, g.VH, vy);
+            } else {
+                vy -= y_move;
+                y = pct_edge(g.Y0, vy, g.VH, PCT_Z_INC, y_move);
+                x = pct_corr(dx_dy, y, y_start, x_start);
+                x_to_go = pct_dist(g.X0, x, PCT_X_INC, x_move, g.VW, vx);
+                y_to_go = g.VH;
+            }
+            if (x_to_go == 0) { x_to_go = g.VW; vx += x_move; }
+            if (y_to_go == 0) { y_to_go = g.VH; vy -= y_move; }
+        }
+        voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
+        outside = (vx >= C) || (vy >= R) || (vz >= S) || (vx < 0) || (vy < 0) || (vz < 0);
+        if (!outside) hit = (hull[voxel] == 1);
+        end_walk = outside || hit;
+        if (++steps > cap) {
+            hit = 0;
+            end_walk = true;
+        }
+    }
+    if (hit) {
+        *xo = x;
+        *yo = y;
+        *zo = z;
+    }
+    *vxo = vx;
+    *vyo = vy;
+    *vzo = vz;
+    return hit;
+}
+
+struct H10 {
+    float* h[10];
+};
+
+// collect_MLP_endpoints_GPU_nobool, DS.cu:7819-7854
+__global__ void k_endpoints(int64_t n, DevGeom g, const uint8_t* __restrict__ hull, H10 hp, uint32_t* first_voxel,
+                            uint8_t* keep)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi = 0, yi = 0, zi = 0, xo = 0, yo = 0, zo = 0;
+    int vx, vy, vz, vxi, vyi, vzi;
+    int entered = find_endpoint(g, hull, hp.h[0][i], hp.h[1][i], hp.h[2][i], hp.h[6][i], hp.h[7][i], &xi, &yi, &zi,
+                                &vx, &vy, &vz, 1);
+    int exited = find_endpoint(g, hull, hp.h[3][i], hp.h[4][i], hp.h[5][i], hp.h[8][i], hp.h[9][i], &xo, &yo, &zo,
+                               &vxi, &vyi, &vzi, 0);
+    if (entered && exited) {
+        first_voxel[i] = (uint32_t)(vx + g.C * vy + g.R * g.C * vz);
+        hp.h[0][i] = (float)xi; hp.h[1][i] = (float)yi; hp.h[2][i] = (float)zi;
+        hp.h[3][i] = (float)xo; hp.h[4][i] = (float)yo; hp.h[5][i] = (float)zo;
+        keep[i] = 1;
+    } else {
+        first_voxel[i] = (uint32_t)g.NV;
+        keep[i] = 0;
+    }
+}
+
+struct MlpArgs {
+    DevGeom g;
+    double trig_min, trig_step, step, coeff_step, poly_step;
+    int64_t n_trig, n_poly, n_coeff;
+    unsigned s_step, p_step;
+    int n_steps;
+};
+
+// Orientation and depth of a history (DS.cu:9120-9147): rotate by the tabulated sin/cos of the
+// entry angle, adding pi to both angles when u_in > u_out.
+struct MlpFrame {
+    double sin_t, cos_t, u_in, u_out, t_in, t_out, xy_entry, xy_exit, u_2;
+};
+__device__ __forceinline__ MlpFrame mlp_frame(const MlpArgs& a, const double* __restrict__ sin_t,
+                                              const double* __restrict__ cos_t, double x_in, double y_in,
+                                              double x_out, double y_out, double xy_entry, double xy_exit)
+{
+    MlpFrame f;
+    int64_t ti = (int64_t)(unsigned int)((xy_entry - a.trig_min) / a.trig_step + 0.5);
+    ti = ti < a.n_trig ? ti : a.n_trig - 1;
+    f.sin_t = sin_t[ti];
+    f.cos_t = cos_t[ti];
+    f.u_in = f.cos_t * x_in + f.sin_t * y_in;
+    f.u_out = f.cos_t * x_out + f.sin_t * y_out;
+    if (f.u_in > f.u_out) {
+        xy_entry += a.g.PI;
+        xy_exit += a.g.PI;
+        ti = (int64_t)(unsigned int)((xy_entry - a.trig_min) / a.trig_step + 0.5);
+        ti = ti < a.n_trig ? ti : a.n_trig - 1;
+        f.sin_t = sin_t[ti];
+        f.cos_t = cos_t[ti];
+        f.u_in = f.cos_t * x_in + f.sin_t * y_in;
+        f.u_out = f.cos_t * x_out + f.sin_t * y_out;
+    }
+    f.t_in = f.cos_t * y_in - f.sin_t * x_in;
+    f.t_out = f.cos_t * y_out - f.sin_t * x_out;
+    f.xy_entry = xy_entry;
+    f.xy_exit = xy_exit;
+    f.u_2 = fabs(f.u_out - f.u_in);
+    return f;
+}
+
+// EffectiveChordLength_GPU, DS.cu:8873-8894
+__device__ double chord_length(double PI, double VW, double VT, double STEP, double abs_angle_t, double abs_angle_v)
+{
+    double eff_t = abs_angle_t - ((int)(abs_angle_t / (PI / 2))) * (PI / 2);
+    double eff_v = fabs(abs_angle_v);
+    double sf = STEP / VW;
+    float s2t = pct_sinf(2 * eff_t), ct = pct_cosf(eff_t), st = pct_sinf(eff_t);
+    float cs = ct + st;
+    double c2d = (1 / 3.0) * ((sf * sf * s2t - 6) / (sf * s2t - (double)(2 * cs)) + sf * sf * s2t / (double)(2 * cs));
+    double mpw = VW / (double)cs;
+    double hf = VT / mpw;
+    sf = STEP / mpw;
+    float s2v = pct_sinf(2 * eff_v), cv = pct_cosf(eff_v), sv = pct_sinf(eff_v);
+    double den = hf * cv + sv;
+    double c3d = (1 / 3.0) * ((sf * sf * s2v - 6 * hf) / (sf * s2v - 2 * den) + sf * sf * s2v / (2 * den));
+    return VW * c2d * c3d;
+}
+
+// capacity of each history's list: 1 + the number of u-steps (loop while depth > STEP)
+__global__ void k_mlp_capacity(int64_t n, MlpArgs a, H10 hp, const double* __restrict__ sin_t,
+                               const double* __restrict__ cos_t, int64_t* cap)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    MlpFrame f = mlp_frame(a, sin_t, cos_t, hp.h[0][i], hp.h[1][i], hp.h[3][i], hp.h[4][i], hp.h[6][i], hp.h[8][i]);
+    int64_t steps = 0;
+    double depth = f.u_2 - a.step;
+    while (depth > a.step && steps < 4 * a.n_steps) {
+        depth -= a.step;
+        steps++;
+    }
+    cap[i] = 1 + steps;
+}
+
+// find_MLP_path_GPU_tabulated, DS.cu:9104-9262, writing the voxel list of history i.
+__global__ void __launch_bounds__(kBlock) k_mlp_lists(int64_t n, MlpArgs a, H10 hp, const uint32_t* __restrict__ first,
+                                                      const double* __restrict__ sin_t,
+                                                      const double* __restrict__ cos_t,
+                                                      const double* __restrict__ scat, const double* __restrict__ p12,
+                                                      const double* __restrict__ p26, const double* __restrict__ p312,
+                                                      const double* __restrict__ step_terms,
+                                                      const int64_t* __restrict__ off, uint32_t* __restrict__ cnt_out,
+                                                      uint32_t* __restrict__ vox, double* __restrict__ chord,
+                                                      int32_t* overflow)
+{
+    extern __shared__ double lds_terms[];  // n_steps x 6
+    for (int k = threadIdx.x; k < a.n_steps; k += blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) lds_terms[k * 6 + c] = step_terms[k * 8 + c];
+    }
+    __syncthreads();
+    const DevGeom& g = a.g;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double x_in = hp.h[0][i], y_in = hp.h[1][i], z_in = hp.h[2][i];
+        const double x_out = hp.h[3][i], y_out = hp.h[4][i], z_out = hp.h[5][i];
+        const double xz_entry = hp.h[7][i], xz_exit = hp.h[9][i];
+        MlpFrame f = mlp_frame(a, sin_t, cos_t, x_in, y_in, x_out, y_out, hp.h[6][i], hp.h[8][i]);
+        const double T2_1 = f.xy_exit - f.xy_entry, V2_1 = xz_exit - xz_entry;
+        const double u_2 = f.u_2, STEP = a.step;
+        double depth = u_2 - STEP;
+        double u_shifted = f.u_in;
+        unsigned s2 = (unsigned)(depth / a.coeff_step + 0.5);
+        int64_t u2i = (int64_t)(unsigned)(u_2 / a.poly_step + 0.5);
+        u2i = u2i < a.n_poly ? u2i : a.n_poly - 1;
+        const double u2_312 = p312[u2i], u2_26 = p26[u2i], u2_12 = p12[u2i];
+        const double u2_sq = (double)pct_sqf(u_2);
+        const int64_t base = off[i], capn = off[i + 1] - off[i];
+        uint32_t last = first[i];
+        int64_t cnt = 0;
+        vox[base + cnt++] = last;
+        chord[i] = chord_length(g.PI, g.VW, g.VT, STEP, (f.xy_entry + f.xy_exit) / 2.0, (xz_entry + xz_exit) / 2.0);
+        int k = 0;
+        while (depth > STEP) {
+            if (k >= a.n_steps || (int64_t)s2 >= a.n_coeff) {
+                atomicOr(overflow, 1);
+                break;
+            }
+            const double* row = lds_terms + k * 6;
+            const double u1_12 = row[0], u1_23 = row[1], u1_34 = row[2];
+            const double S1I0 = row[3], S1I1 = row[4], S1I2 = row[5];
+            double st2 = u2_312 - u2_sq * u1_12 + 2 * u_2 * u1_23 - u1_34;
+            double st2th2 = u2_26 - u_2 * u1_12 + u1_23;
+            double sth2 = u2_12 - u1_12;
+            double det2 = scat[s2] * (st2 * sth2 - (double)pct_sqf(st2th2));
+            double S2I0 = sth2 / det2, S2I1 = st2th2 / det2, S2I2 = st2 / det2;
+            double c24 = S2I0 * depth - S2I1;
+            double ft0 = S1I0 + S2I0;
+            double ft1 = c24 - S1I1;
+            double ft2 = depth * S2I0 - S1I1 - S2I1;
+            double ft3 = S1I2 + S2I2 + depth * (c24 - S2I1);
+            double detft = ft0 * ft3 - ft1 * ft2;
+            double c3 = S2I0 * f.t_out - S2I1 * T2_1;
+            double sc0 = S1I0 * f.t_in + c3;
+            double sc1 = depth * c3 + S2I2 * T2_1 - S2I1 * f.t_out - S1I1 * f.t_in;
+            double t_1 = (ft3 * sc0 - ft1 * sc1) / detft;
+            c3 = S2I0 * z_out - S2I1 * V2_1;
+            sc0 = S1I0 * z_in + c3;
+            sc1 = depth * c3 + S2I2 * V2_1 - S2I1 * z_out - S1I1 * z_in;
+            double v_1 = (ft3 * sc0 - ft1 * sc1) / detft;
+            u_shifted += STEP;
+            double x_1 = f.cos_t * u_shifted - f.sin_t * t_1;
+            double y_1 = f.sin_t * u_shifted + f.cos_t * t_1;
+            int vx = pct_calc_voxel(g.X0, x_1, g.VW);
+            int vy = pct_calc_voxel(g.Y0, y_1, g.VH);
+            int vz = pct_calc_voxel(g.Z0, v_1, g.VT);
+            int64_t voxel = vx + (int64_t)vy * g.C + (int64_t)vz * g.C * g.R;
+            if (voxel != (int64_t)last && voxel < g.NV) {
+                if (cnt >= capn) {
+                    atomicOr(overflow, 2);
+                    break;
+                }
+                vox[base + cnt++] = (uint32_t)voxel;
+                last = (uint32_t)voxel;
+            }
+            depth -= STEP;
+            s2 -= a.s_step;
+            k++;
+        }
+        cnt_out[i] = (uint32_t)cnt;
+    }
+}
+
+}  // namespace
+
+// reconstruction_cuts (DS.cu:8296): endpoints on the selected hull, keep histories that hit it
+int pct_stage_endpoints(pct_ctx* ctx)
+{
+    const pct_params& P = ctx->P;
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->n_hist;
+    DevGeom g = pct_geom(P);
+    H10 hp;
+    for (int k = 0; k < 10; k++) hp.h[k] = ctx->h[k];
+    if (n > 0)
+        hipLaunchKernelGGL(k_endpoints, dim3(grid_for(n)), dim3(kBlock), 0, s, n, g, ctx->hull, hp, ctx->first_voxel,
+                           ctx->flag);
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    int64_t kept = 0;
+    PCT_TRY(pct_compact(ctx, ctx->flag, n, &kept, true));  // DS.cu:8286-8294
+    ctx->n_hist = kept;
+    ctx->n_recon = kept;
+    ctx->stage = ST_ENDPOINTS;
+    return PCT_OK;
+}
+
+static int upload_tables(pct_ctx* ctx)
+{
+    if (ctx->T.sin_t) return PCT_OK;
+    const pct_params& P = ctx->P;
+    hipStream_t s = ctx->stream;
+    HostTables ht;
+    pct_build_tables(P, &ht);
+    int n_steps = (int)(2 * P.recon_cyl_radius / P.mlp_u_step) + 8;
+    std::vector<double> terms;
+    pct_build_step_terms(P, ht, n_steps, &terms);
+    DevTables& T = ctx->T;
+    T.n_trig = (int64_t)ht.sin_t.size();
+    T.n_coeff = (int64_t)ht.scat.size();
+    T.n_poly = (int64_t)ht.p12.size();
+    T.n_steps = n_steps;
+    PCT_TRY(pct_alloc(ctx, &T.sin_t, ht.sin_t.size()));
+    PCT_TRY(pct_alloc(ctx, &T.cos_t, ht.cos_t.size()));
+    PCT_TRY(pct_alloc(ctx, &T.scat, ht.scat.size()));
+    PCT_TRY(pct_alloc(ctx, &T.p12, ht.p12.size()));
+    PCT_TRY(pct_alloc(ctx, &T.p26, ht.p26.size()));
+    PCT_TRY(pct_alloc(ctx, &T.p312, ht.p312.size()));
+    PCT_TRY(pct_alloc(ctx, &T.step_terms, terms.size()));
+    auto up = [&](double* d, const std::vector<double>& h) {
+        return hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    };
+    PCT_CHECK_HIP(ctx, up(T.sin_t, ht.sin_t));
+    PCT_CHECK_HIP(ctx, up(T.cos_t, ht.cos_t));
+    PCT_CHECK_HIP(ctx, up(T.scat, ht.scat));
+    PCT_CHECK_HIP(ctx, up(T.p12, ht.p12));
+    PCT_CHECK_HIP(ctx, up(T.p26, ht.p26));
+    PCT_CHECK_HIP(ctx, up(T.p312, ht.p312));
+    PCT_CHECK_HIP(ctx, up(T.step_terms, terms));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    return PCT_OK;
+}
+
+int pct_stage_build_lists(pct_ctx* ctx)
+{
+    const pct_params& P = ctx->P;
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->n_hist;
+    PCT_TRY(upload_tables(ctx));
+    MlpArgs a;
+    a.g = pct_geom(P);
+    a.trig_min = P.trig_table_min;
+    a.trig_step = P.trig_table_step;
+    a.step = P.mlp_u_step;
+    a.coeff_step = P.coeff_table_step;
+    a.poly_step = P.poly_table_step;
+    a.n_trig = ctx->T.n_trig;
+    a.n_poly = ctx->T.n_poly;
+    a.n_coeff = ctx->T.n_coeff;
+    a.s_step = (unsigned)(P.mlp_u_step / P.coeff_table_step + 0.5);
+    a.p_step = (unsigned)(P.mlp_u_step / P.poly_table_step + 0.5);
+    a.n_steps = ctx->T.n_steps;
+    H10 hp;
+    for (int k = 0; k < 10; k++) hp.h[k] = ctx->h[k];
+    PCT_TRY(pct_alloc(ctx, &ctx->list_off, n + 1));
+    PCT_TRY(pct_alloc(ctx, &ctx->list_cnt, n));
+    PCT_TRY(pct_alloc(ctx, &ctx->chord, n));
+    if (!ctx->list_overflow) PCT_TRY(pct_alloc(ctx, &ctx->list_overflow, 1));
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->list_overflow, 0, sizeof(int32_t), s));
+    // capacities -> exclusive scan -> offsets
+    int64_t* capv = nullptr;
+    PCT_TRY(pct_alloc(ctx, &capv, n + 1));
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(capv, 0, (n + 1) * sizeof(int64_t), s));
+    if (n > 0)
+        hipLaunchKernelGGL(k_mlp_capacity, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, hp, ctx->T.sin_t,
+                           ctx->T.cos_t, capv);
+    size_t need = 0;
+    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, capv, ctx->list_off, n + 1, s));
+    if (need > ctx->cub_tmp_bytes) {
+        PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
+        ctx->cub_tmp_bytes = need;
+    }
+    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, need, capv, ctx->list_off, n + 1, s));
+    int64_t total = 0;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&total, ctx->list_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    pct_free(&capv);
+    if (total > ctx->list_cap) {
+        PCT_TRY(pct_alloc(ctx, &ctx->list_vox, total));
+        ctx->list_cap = total;
+    }
+    if (n > 0) {
+        size_t lds = (size_t)a.n_steps * 6 * sizeof(double);
+        if (lds > 64 * 1024) {
+            ctx->err = "MLP: too many u-steps for the LDS step table";
+            return PCT_E_LIMIT;
+        }
+        int dev_cus = 256;
+        (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+        int64_t want = (n + kBlock - 1) / kBlock;
+        int64_t cap_blocks = (int64_t)dev_cus * 8;
+        unsigned blocks = (unsigned)(want < cap_blocks ? want : cap_blocks);
+        hipLaunchKernelGGL(k_mlp_lists, dim3(blocks), dim3(kBlock), lds, s, n, a, hp, ctx->first_voxel, ctx->T.sin_t,
+                           ctx->T.cos_t, ctx->T.scat, ctx->T.p12, ctx->T.p26, ctx->T.p312, ctx->T.step_terms,
+                           ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord, ctx->list_overflow);
+        PCT_CHECK_HIP(ctx, hipGetLastError());
+    }
+    int32_t overflow = 0;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&overflow, ctx->list_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    if (overflow) {
+        ctx->err = "MLP list capacity exceeded (code " + std::to_string(overflow) + ")";
+        return PCT_E_LIMIT;
+    }
+    ctx->list_entries = total;
+    ctx->stage = ST_LISTS;
+    return PCT_OK;
+}

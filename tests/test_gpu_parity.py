@@ -1,0 +1,102 @@
+"""Parity of the gfx950 pipeline with the CPU oracle, stage by stage, on seeded inputs.
+
+Bar: bit-exact for history counts, bin numbers, per-bin statistics and cut decisions, hull
+counts and masks, FBP images, MLP endpoints and voxel lists; DROP images within 1e-6 relative L2
+of the oracle's FP64-accumulated iteration and 1e-4 of its FP32 (reference-precision) iteration.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cases import small_cylinder, small_head
+
+pytestmark = pytest.mark.gpu
+
+H_NAMES = ["x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "xy_entry_angle", "xz_entry_angle",
+           "xy_exit_angle", "xz_exit_angle"]
+
+
+def _mismatch(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    if a.dtype.kind == "f":
+        return int(np.sum(~((a == b) | (np.isnan(a) & np.isnan(b)))))
+    return int(np.sum(a != b))
+
+
+def _run_and_compare(P, data, oracle, case):
+    from pct_reconstruction_amd import Reconstruction
+    ref = oracle.run_pipeline(P, data, iterations=P.iterations, accum=1)
+    ref32 = oracle.run_pipeline(P, data, iterations=P.iterations, accum=0)
+    print(f"[{case}] n={data['wepl'].shape[0]} oracle done", flush=True)
+    with Reconstruction(P) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.import_and_process_data()
+        assert r.count("recon_volume") == ref["h_volume"].shape[1]
+        for k, name in enumerate(H_NAMES):
+            assert _mismatch(r.histories(name), ref["h_volume"][k]) == 0, name
+        assert _mismatch(r.histories("bin_num"), ref["bin_num"]) == 0
+        if P.sc_on:
+            assert _mismatch(r.image("sc_hull").reshape(-1), ref["sc_hull"]) == 0
+        if P.msc_on:
+            assert _mismatch(r.image("msc_counts").reshape(-1), ref["msc_counts"]) == 0
+        if P.sm_on:
+            assert _mismatch(r.image("sm_counts").reshape(-1), ref["sm_counts"]) == 0
+        r.hull_detection_finish()
+        if P.msc_on:
+            assert _mismatch(r.image("msc_hull").reshape(-1), ref["msc_hull"]) == 0
+        if P.sm_on:
+            assert _mismatch(r.image("sm_hull").reshape(-1), ref["sm_hull"]) == 0
+        r.statistical_calculations_and_cuts()
+        assert _mismatch(r.bins("counts_pre"), ref["counts_pre"]) == 0
+        for k, nm in enumerate(["wepl", "ut", "uv"]):
+            assert _mismatch(r.bins("mean_" + nm), ref["mean"][k]) == 0, nm
+            assert _mismatch(r.bins("std_" + nm), ref["std"][k]) == 0, nm
+        assert _mismatch(r.bins("counts_post"), ref["counts_post"]) == 0
+        assert _mismatch(r.bins("sinogram"), ref["sinogram_sum"]) == 0
+        assert r.count("post_cuts") == ref["h_cut"].shape[1]
+        for k, name in enumerate(H_NAMES):
+            assert _mismatch(r.histories(name), ref["h_cut"][k]) == 0, name
+        r.generate_preprocessing_images()
+        assert _mismatch(r.bins("sinogram"), ref["sinogram"]) == 0
+        assert _mismatch(r.bins("sinogram_filtered"), ref["sinogram_filtered"]) == 0
+        fbp = r.image("fbp").reshape(-1)
+        n_fbp = _mismatch(fbp, ref["fbp"])
+        print(f"[{case}] fbp mismatches {n_fbp} max abs {np.max(np.abs(fbp - ref['fbp']))}", flush=True)
+        assert n_fbp == 0
+        assert _mismatch(r.image("hull").reshape(-1), ref["hull"].astype(np.float32)) == 0
+        assert _mismatch(r.image("x0").reshape(-1), ref["x0"]) == 0
+        r.reconstruction_cuts()
+        assert r.count("reconstruction") == ref["h_recon"].shape[1]
+        assert _mismatch(r.histories("first_voxel"), ref["first_voxel"]) == 0
+        for k, name in enumerate(H_NAMES[:6]):
+            assert _mismatch(r.histories(name), ref["h_recon"][k]) == 0, name
+        r.build_mlp_lists()
+        off, vox = r.lists()
+        assert _mismatch(off, ref["offsets"]) == 0
+        assert _mismatch(vox, ref["voxels"]) == 0
+        chord = r.histories("chord")
+        np.testing.assert_allclose(chord, ref["chord"], rtol=1e-12, atol=0)
+        r.DROP(P.iterations)
+        x = r.image("x").reshape(-1)
+        rel64 = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
+        rel32 = np.linalg.norm(x - ref32["x"]) / np.linalg.norm(ref32["x"])
+        print(f"[{case}] DROP rel L2 vs FP64-accum oracle {rel64:.3e}, vs FP32 oracle {rel32:.3e}", flush=True)
+        assert rel64 <= 1e-6
+        assert rel32 <= 1e-4
+        res = r.residual()
+        ref_res = oracle.residual(ref["offsets"], ref["voxels"], ref["chord"], ref["wepl_recon"], ref["x"],
+                                  P.min_mlp_length)
+        assert abs(res - ref_res) <= 1e-6 * ref_res
+
+
+def test_parity_cylinder_sc(oracle):
+    P, data = small_cylinder()
+    _run_and_compare(P, data, oracle, "cylinder-SC")
+
+
+def test_parity_head_msc(oracle):
+    P, data = small_head()
+    _run_and_compare(P, data, oracle, "head-MSC")
