@@ -1,0 +1,185 @@
+"""MLP+DROP throughput on one MI355X: the head-50M workload.
+
+Workload: 10-ellipsoid head phantom, 50M seeded synthetic histories with
+continuous gantry angles, 256x256x96 grid at ~1 mm, MSC + SM hulls, 3-sigma cuts, FBP x0, MLP + DROP
+for 20 iterations with block 60000.  Preprocessing (binning, cuts, hulls, FBP, endpoints) runs once
+before the timed region; one timed step = building the MLP lists of all reconstruction histories
+plus 20 DROP iterations from x0, all inputs resident in HBM.
+
+metric: MLP+DROP proton histories/s = reconstruction histories x DROP iterations / step time.
+The JSON line also carries the roofline of the dominant kernel (HIP-event timing of every launch of
+that kernel during one profiled iteration) and the oracle's CPU rate on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(P, r, sample: int, threads: int, iterations: int) -> dict:
+    """Oracle (C, OpenMP) MLP lists + one DROP iteration on the first `sample` reconstruction
+    histories, extrapolated per history-iteration like the GPU metric."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    import pct_oracle as O
+    names = ["x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "xy_entry_angle", "xz_entry_angle",
+             "xy_exit_angle", "xz_exit_angle"]
+    h = np.ascontiguousarray(np.stack([r.histories(k)[:sample] for k in names]))
+    first = np.ascontiguousarray(r.histories("first_voxel")[:sample])
+    w = np.ascontiguousarray(r.histories("wepl")[:sample])
+    x0 = r.image("x0").reshape(-1)
+    tabs = O.MlpTables(P)
+    t0 = time.perf_counter()
+    off, vox, chord = O.mlp_lists(P, tabs, h, first)
+    t_mlp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.drop_iteration(P, off, vox, chord, w, x0, accum=0, nthreads=threads)
+    t_drop = time.perf_counter() - t0
+    n = h.shape[1]
+    rate = n * iterations / (t_mlp + iterations * t_drop)
+    return {"value": rate, "unit": "histories/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} reconstruction histories of the workload: oracle MLP lists ({t_mlp:.2f} s) + "
+                      f"one DROP iteration ({t_drop:.2f} s), rate = n*{iterations}/(t_mlp + {iterations}*t_drop)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--histories", type=int, default=50_000_000)
+    ap.add_argument("--iterations", type=int, default=None, help="DROP iterations per step (default: config)")
+    ap.add_argument("--cpu-sample", type=int, default=120_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not implemented)"}))
+        return 2
+
+    from pct_reconstruction_amd import Reconstruction, configs, synth
+
+    P, ph, bm, n = configs.head_50m(args.histories)
+    iters = P.iterations if args.iterations is None else args.iterations
+    log(f"generating {n} histories (seed {args.seed})")
+    t0 = time.perf_counter()
+    data = synth.generate(ph, bm, args.seed, n)
+    log(f"generated in {time.perf_counter() - t0:.1f} s")
+
+    r = Reconstruction(P, device=0)
+    r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+    del data
+    log("loaded; preprocessing")
+    t0 = time.perf_counter()
+    r.preprocessing()
+    r.reconstruction_cuts()
+    r.synchronize()
+    t_pre = time.perf_counter() - t0
+    m = r.metrics()
+    log(f"preprocessing {t_pre:.2f} s; counts {m['counts']}")
+    n_recon = r.count("reconstruction")
+
+    def step():
+        r.reset_iterate()
+        r.build_mlp_lists()
+        r.DROP(iters)
+
+    for k in range(args.warmup):
+        step()
+        r.synchronize()
+        log(f"warmup step {k + 1}/{args.warmup} done")
+    r.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step()
+    r.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"{args.steps} timed steps in {elapsed:.3f} s")
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = n_recon * iters * args.steps / elapsed
+    m = r.metrics()
+    entries = r.count("list_entries")
+
+    # per-kernel timing: one profiled DROP iteration (every launch bracketed by HIP events)
+    r.reset_iterate()
+    prof = r.profile_drop(1)
+    mlp_ms = m["kernel_ms"]["mlp_lists"]
+    drop_bytes_iter = 8 * prof["gathered"] + 24 * prof["scattered"] + 24 * prof["histories"]
+    scatter_bytes = drop_bytes_iter / max(prof["launches"], 1)
+    scatter_gbs = scatter_bytes / (prof["scatter_ms"] * 1e-3) / 1e9
+    mlp_bytes = 4 * entries + 64 * n_recon
+    mlp_gbs = mlp_bytes / (mlp_ms * 1e-3) / 1e9
+    drop_total_ms = iters * prof["launches"] * prof["scatter_ms"]
+    kernels = {
+        "drop_scatter": {"avg_launch_ms": prof["scatter_ms"], "launches_per_iteration": prof["launches"],
+                         "bytes_per_launch": scatter_bytes, "GBs": scatter_gbs,
+                         "ms_per_step": drop_total_ms},
+        "drop_apply": {"avg_launch_ms": prof["apply_ms"], "ms_per_step": iters * prof["launches"] * prof["apply_ms"]},
+        "mlp_lists": {"avg_launch_ms": mlp_ms, "bytes_per_launch": mlp_bytes, "GBs": mlp_gbs, "ms_per_step": mlp_ms},
+    }
+    dominant = "drop_scatter" if drop_total_ms >= mlp_ms else "mlp_lists"
+    dk = kernels[dominant]
+    traffic = None
+    tfile = ROOT / "profiles" / f"{dominant}_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": dk["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": dk["GBs"] / HBM_PEAK_GBS, "traffic": traffic}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"cpu baseline on {args.cpu_sample} histories, {threads} threads")
+        cpu = cpu_baseline(r.params, r, min(args.cpu_sample, n_recon), threads, iters)
+
+    stages = {k: round(v, 3) for k, v in m["stage_ms"].items()}
+    out = {
+        "metric": "MLP+DROP proton histories/s",
+        "value": value,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (MLP, accumulators) / f32 (image)",
+        "data": "synthetic (seeded 10-ellipsoid head phantom, Highland scatter, WEPL noise 2.5 mm, 1% outliers)",
+        "config": {"workload": "head-50M: 256x256x96 @ ~1 mm, MSC+SM hulls, 3-sigma cuts, FBP x0, "
+                               f"MLP + DROP {iters} iterations, block {P.drop_block_size}",
+                   "histories_loaded": n, "histories_reconstructed": n_recon, "list_entries": entries,
+                   "drop_iterations": iters, "drop_block_size": P.drop_block_size},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "stages_ms": stages,
+        "preprocessing_s": t_pre,
+        "counts": m["counts"],
+    }
+    print(json.dumps(out))
+    r.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

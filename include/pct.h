@@ -36,8 +36,9 @@ extern "C" {
 enum pct_hull_type { PCT_HULL_IMPORT = 0, PCT_HULL_SC = 1, PCT_HULL_MSC = 2, PCT_HULL_SM = 3, PCT_HULL_FBP = 4 };
 enum pct_filter_type { PCT_FILTER_RAM_LAK = 0, PCT_FILTER_SHEPP_LOGAN = 1, PCT_FILTER_NONE = 2 };
 enum pct_x0_type { PCT_X0_IMPORT = 0, PCT_X0_HULL = 1, PCT_X0_FBP = 2, PCT_X0_HYBRID = 3, PCT_X0_ZEROS = 4 };
-/* DROP per-voxel accumulation precision (the reference uses float atomicAdd, DS.cu:9533). */
-enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1 };
+/* DROP per-voxel accumulation precision (the reference uses float atomicAdd, DS.cu:9533):
+ * FP32 sums in history order (reference precision), FP64 sums, or FP32 with Kahan compensation. */
+enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1, PCT_ACCUM_KAHAN32 = 2 };
 
 /*
  * Parameters.  Field names follow the settings.cfg keys (settings.cfg:1-164) and the
@@ -162,6 +163,14 @@ int pct_mlp_build_lists(pct_ctx* ctx);
 int pct_drop_iterate(pct_ctx* ctx, int iterations);
 /* Run every stage above in the reference order, with params.iterations DROP iterations. */
 int pct_run_pipeline(pct_ctx* ctx);
+/* Reset the iterate x to the initial iterate x0 (on the device). */
+int pct_reset_iterate(pct_ctx* ctx);
+/* Run `iterations` DROP iterations with a HIP event pair around every kernel launch and report the
+ * average duration of one block's scatter kernel and apply kernel (ms), the number of blocks
+ * launched, and per iteration the histories processed, list entries gathered and list entries
+ * scattered.  Updates x like pct_drop_iterate. */
+int pct_drop_profile(pct_ctx* ctx, int iterations, double* scatter_ms, double* apply_ms, int64_t* launches,
+                     int64_t* histories, int64_t* gathered, int64_t* scattered);
 
 /* ---------------------------------------------------------------- outputs ---- */
 enum pct_image_kind { PCT_IMG_X = 0, PCT_IMG_FBP = 1, PCT_IMG_X0 = 2, PCT_IMG_HULL = 3,

@@ -904,16 +904,50 @@ static int64_t o_mlp_path(const pct_params* p, const o_tables* tb, const float* 
 int64_t o_mlp_lists(const pct_params* p, const o_tables* t, int64_t n, const float* const h[10],
                     const uint32_t* first_voxel, int64_t* offsets, uint32_t* voxels, int64_t capacity, double* chord)
 {
-    int64_t total = 0;
-    offsets[0] = 0;
-    for (int64_t i = 0; i < n; i++) {
-        double L;
-        int64_t c = o_mlp_path(p, t, h, i, first_voxel[i], voxels + total, capacity - total, &L);
-        if (c < 0) return -1;
-        total += c;
-        offsets[i + 1] = total;
-        if (chord) chord[i] = L;
+    /* Histories are split into contiguous chunks; each chunk's lists go to a private buffer that is
+     * concatenated in history order afterwards, so the result does not depend on the thread count. */
+    int nchunks = 1;
+#ifdef _OPENMP
+    nchunks = omp_get_max_threads() * 4;
+#endif
+    if (n < 4096) nchunks = 1;
+    const int64_t per_path = (int64_t)(2 * p->recon_cyl_radius / p->mlp_u_step) + 8;
+    int64_t* csize = (int64_t*)calloc((size_t)nchunks, sizeof(int64_t));
+    uint32_t** cbuf = (uint32_t**)calloc((size_t)nchunks, sizeof(uint32_t*));
+    int failed = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < nchunks; c++) {
+        int64_t a = n * c / nchunks, b = n * (c + 1) / nchunks;
+        int64_t cap = (b - a) * per_path + 1;
+        uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)cap);
+        int64_t used = 0;
+        for (int64_t i = a; i < b && buf; i++) {
+            double L;
+            int64_t k = o_mlp_path(p, t, h, i, first_voxel[i], buf + used, cap - used, &L);
+            if (k < 0) { failed = 1; break; }
+            used += k;
+            offsets[i + 1] = k;   /* count for now */
+            if (chord) chord[i] = L;
+        }
+        if (!buf) failed = 1;
+        cbuf[c] = buf;
+        csize[c] = used;
     }
+    int64_t total = 0;
+    for (int c = 0; c < nchunks; c++) total += csize[c];
+    if (failed || total > capacity) total = -1;
+    if (total >= 0) {
+        offsets[0] = 0;
+        for (int64_t i = 0; i < n; i++) offsets[i + 1] += offsets[i];
+        int64_t pos = 0;
+        for (int c = 0; c < nchunks; c++) {
+            memcpy(voxels + pos, cbuf[c], sizeof(uint32_t) * (size_t)csize[c]);
+            pos += csize[c];
+        }
+    }
+    for (int c = 0; c < nchunks; c++) free(cbuf[c]);
+    free(cbuf);
+    free(csize);
     return total;
 }
 
@@ -932,6 +966,7 @@ void o_drop_iteration(const pct_params* p, int64_t n, const int64_t* offsets, co
     const int64_t NV = p->num_voxels;
     const double lambda = p->lambda;
     float* U32 = (float*)calloc((size_t)NV, sizeof(float));
+    float* K32 = (float*)calloc((size_t)NV, sizeof(float)); /* Kahan compensation */
     double* U64 = (double*)calloc((size_t)NV, sizeof(double));
     uint32_t* S = (uint32_t*)calloc((size_t)NV, sizeof(uint32_t));
     for (int64_t start = 0; start < n; start += p->drop_block_size) {                   /* :9549-9582 */
@@ -949,40 +984,50 @@ void o_drop_iteration(const pct_params* p, int64_t n, const int64_t* offsets, co
                 if (cnt >= p->min_mlp_length) {                                         /* :9522-9536 */
                     double val = L * ((wepl[i] - ax) / aa) * lambda;
                     for (int64_t j = 0; j < cnt; j++) {
-                        S[a[j]] += 1;
-                        if (accum == PCT_ACCUM_FP64) U64[a[j]] += val;
-                        else U32[a[j]] += (float)val;
+                        uint32_t v = a[j];
+                        S[v] += 1;
+                        if (accum == PCT_ACCUM_FP64) U64[v] += val;
+                        else if (accum == PCT_ACCUM_KAHAN32) {
+                            float y = (float)val - K32[v];
+                            float t = U32[v] + y;
+                            K32[v] = (t - U32[v]) - y;
+                            U32[v] = t;
+                        } else U32[v] += (float)val;
                     }
                 }
             }
         } else {
 #ifdef _OPENMP
-#pragma omp parallel num_threads(nthreads)
-            {
-                float* lu = (float*)calloc((size_t)NV, sizeof(float));
-                uint32_t* ls = (uint32_t*)calloc((size_t)NV, sizeof(uint32_t));
-#pragma omp for schedule(static)
-                for (int64_t i = start; i < start + proc; i++) {
-                    int64_t cnt = offsets[i + 1] - offsets[i];
-                    const uint32_t* a = voxels + offsets[i];
-                    double L = o_row_chord(chord, i);
-                    double ax = 0.0;
-                    for (int64_t j = 0; j < cnt; j++) ax += x[a[j]];
-                    ax *= L;
-                    double aa = (double)f_sq(L) * cnt;
-                    if (cnt >= p->min_mlp_length) {
-                        double val = L * ((wepl[i] - ax) / aa) * lambda;
-                        for (int64_t j = 0; j < cnt; j++) { ls[a[j]] += 1; lu[a[j]] += (float)val; }
+            /* CPU baseline: histories of a block in parallel, float atomic sums (order varies) */
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+            for (int64_t i = start; i < start + proc; i++) {
+                int64_t cnt = offsets[i + 1] - offsets[i];
+                const uint32_t* a = voxels + offsets[i];
+                double L = o_row_chord(chord, i);
+                double ax = 0.0;
+                for (int64_t j = 0; j < cnt; j++) ax += x[a[j]];
+                ax *= L;
+                double aa = (double)f_sq(L) * cnt;
+                if (cnt >= p->min_mlp_length) {
+                    double val = L * ((wepl[i] - ax) / aa) * lambda;
+                    float fv = (float)val;
+                    for (int64_t j = 0; j < cnt; j++) {
+#pragma omp atomic
+                        S[a[j]] += 1;
+#pragma omp atomic
+                        U32[a[j]] += fv;
                     }
                 }
-#pragma omp critical
-                {
-                    for (int64_t v = 0; v < NV; v++)
-                        if (ls[v]) { S[v] += ls[v]; U32[v] += lu[v]; }
-                }
-                free(lu);
-                free(ls);
             }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int64_t v = 0; v < NV; v++) {
+                if (S[v] > 0) {
+                    x[v] += U32[v] / (float)S[v];
+                    U32[v] = 0.0f;
+                    S[v] = 0;
+                }
+            }
+            continue;
 #endif
         }
         for (int64_t v = 0; v < NV; v++) {                                               /* :9398-9419 */
@@ -990,11 +1035,13 @@ void o_drop_iteration(const pct_params* p, int64_t n, const int64_t* offsets, co
                 if (accum == PCT_ACCUM_FP64) x[v] += (float)(U64[v] / (double)S[v]);
                 else x[v] += U32[v] / (float)S[v];
                 U32[v] = 0.0f;
+                K32[v] = 0.0f;
                 U64[v] = 0.0;
                 S[v] = 0;
             }
         }
     }
+    free(K32);
     free(U32);
     free(U64);
     free(S);

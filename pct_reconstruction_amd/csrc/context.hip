@@ -130,10 +130,13 @@ int pct_ctx_destroy(pct_ctx* ctx)
     pct_free(&ctx->msc_hull); pct_free(&ctx->sm_hull); pct_free(&ctx->fbp_hull); pct_free(&ctx->hull);
     pct_free(&ctx->hull_import); pct_free(&ctx->fbp); pct_free(&ctx->x0); pct_free(&ctx->x);
     pct_free(&ctx->list_off); pct_free(&ctx->list_cnt); pct_free(&ctx->list_vox); pct_free(&ctx->chord);
+    pct_free(&ctx->cap_scratch);
     pct_free(&ctx->T.sin_t); pct_free(&ctx->T.cos_t); pct_free(&ctx->T.scat); pct_free(&ctx->T.p12);
     pct_free(&ctx->T.p26); pct_free(&ctx->T.p312); pct_free(&ctx->T.step_terms);
     pct_free(&ctx->accum64); pct_free(&ctx->accum32); pct_free(&ctx->S); pct_free(&ctx->touched);
     pct_free(&ctx->n_touched); pct_free(&ctx->list_overflow);
+    pct_free(&ctx->plan.t_vox); pct_free(&ctx->plan.t_off); pct_free(&ctx->plan.t_hist); pct_free(&ctx->plan.n_vox);
+    pct_free(&ctx->plan.r);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PCT_OK;
@@ -506,9 +509,19 @@ const char* pct_metrics_json(pct_ctx* ctx)
         std::snprintf(buf, sizeof(buf), "%s\"%s\": %.4f", k ? ", " : "", kTimerName[k], ctx->t_stage[k]);
         s += buf;
     }
-    s += "}}";
+    std::snprintf(buf, sizeof(buf), "}, \"kernel_ms\": {\"mlp_lists\": %.4f}}", ctx->t_mlp_kernel);
+    s += buf;
     ctx->metrics = s;
     return ctx->metrics.c_str();
+}
+
+int pct_reset_iterate(pct_ctx* ctx)
+{
+    if (!ctx) return PCT_E_ARG;
+    PCT_TRY(require(ctx, ST_X0, "pct_reset_iterate"));
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->x, ctx->x0, ctx->P.num_voxels * sizeof(float), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+    return PCT_OK;
 }
 
 int pct_drop_iteration_device(int64_t n, const int64_t* offsets, const uint32_t* voxels, const double* chord,

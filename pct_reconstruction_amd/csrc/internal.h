@@ -50,6 +50,25 @@ struct DevTables {
     int64_t n_trig = 0, n_coeff = 0, n_poly = 0;
 };
 
+// Per-block transposed lists for the atomic-free DROP (k_drop2.hip).  Block b covers histories
+// [start, start + proc); its `entries` list entries (histories with >= MIN_MLP_LENGTH voxels) are
+// grouped by voxel: t_vox[u] touched voxel, t_off[u..u+1] its range in t_hist (history index
+// within the block, ascending).  Arrays of all blocks are concatenated at entry_base
+// (t_off at entry_base + b, one extra element per block).
+struct DropPlan {
+    struct Block {
+        int64_t start, proc, entry_base, entries;
+    };
+    std::vector<Block> blocks;
+    uint32_t* t_vox = nullptr;
+    uint32_t* t_off = nullptr;
+    uint32_t* t_hist = nullptr;
+    uint32_t* n_vox = nullptr;  // per block: number of touched voxels
+    double* r = nullptr;        // per history of the current block: its update value
+    int64_t cap_entries = 0, cap_off = 0, cap_blocks = 0, cap_r = 0;
+    bool ready = false;
+};
+
 struct pct_ctx {
     pct_params P;
     int device = 0;
@@ -112,10 +131,12 @@ struct pct_ctx {
     uint32_t* list_cnt = nullptr;  // n_recon
     uint32_t* list_vox = nullptr;
     double* chord = nullptr;       // n_recon
-    int64_t list_cap = 0, list_entries = 0;
+    int64_t* cap_scratch = nullptr;  // n_recon + 1
+    int64_t list_cap = 0, list_entries = 0, list_hist_cap = 0;
     DevTables T;
 
-    // DROP scratch
+    // DROP plan (transposed lists) and the scratch of the atomic DROP kernels
+    DropPlan plan;
     double* accum64 = nullptr;
     float* accum32 = nullptr;
     uint32_t* S = nullptr;
@@ -123,8 +144,9 @@ struct pct_ctx {
     uint32_t* n_touched = nullptr;  // 2 counters (ping-pong)
     int32_t* list_overflow = nullptr;
 
-    // timing (ms) of the last run of each stage
+    // timing (ms) of the last run of each stage, and of the last MLP list kernel launch
     double t_stage[16] = {};
+    double t_mlp_kernel = 0;
 };
 
 // allocation helpers
@@ -170,13 +192,23 @@ int pct_stage_select_hull(pct_ctx* ctx);
 int pct_stage_initial_iterate(pct_ctx* ctx, const float* x0_import);
 int pct_stage_endpoints(pct_ctx* ctx);
 int pct_stage_build_lists(pct_ctx* ctx);
-int pct_stage_drop(pct_ctx* ctx, int iterations);
+struct DropTiming;
+int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing = nullptr);
+int pct_drop_entries(pct_ctx* ctx, int64_t* histories, int64_t* gathered, int64_t* scattered);
+int pct_drop_prepare(pct_ctx* ctx);
+int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing);
 int pct_stage_residual(pct_ctx* ctx, double* norm);
 
 // compaction: keep entries with flag != 0 (flag array has ctx->n_hist entries); returns new count
 int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bool with_first_voxel);
+// per-kernel timing of DROP launches (HIP events around every launch on the launch stream)
+struct DropTiming {
+    double scatter_ms = 0, apply_ms = 0;
+    int64_t scatter_launches = 0, apply_launches = 0;
+    int64_t entries = 0, histories = 0;  // algorithmic work of the timed scatter launches
+};
 // DROP iteration on raw device buffers
 int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* cnt, const uint32_t* vox,
                  const double* chord, const float* wepl, float* x, double* accum64, float* accum32, uint32_t* S,
                  uint32_t* touched, uint32_t* n_touched, int64_t num_voxels, int block_size, int hpb, int quirk_a3,
-                 int min_len, double lambda, int accum_mode, std::string* err);
+                 int min_len, double lambda, int accum_mode, std::string* err, DropTiming* timing = nullptr);

@@ -120,7 +120,7 @@ __global__ void k_drop_apply_dense(int64_t nv, float* __restrict__ x, double* __
 int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* cnt, const uint32_t* vox,
                  const double* chord, const float* wepl, float* x, double* acc64, float* acc32, uint32_t* S,
                  uint32_t* touched, uint32_t* n_touched, int64_t num_voxels, int block_size, int hpb, int quirk_a3,
-                 int min_len, double lambda, int accum_mode, std::string* err)
+                 int min_len, double lambda, int accum_mode, std::string* err, DropTiming* timing)
 {
     const bool fp64 = accum_mode == PCT_ACCUM_FP64;
     const bool use_touched = touched != nullptr && n_touched != nullptr;
@@ -131,6 +131,14 @@ int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* c
             return PCT_E_HIP;
         }
     }
+    // timing: one event pair per launch, read back after the stream drains
+    std::vector<hipEvent_t> ev;
+    auto mark = [&]() {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        ev.push_back(e);
+    };
     int k = 0;
     for (int64_t start = 0; start < n; start += block_size, k++) {
         int64_t m = n - start < block_size ? n - start : block_size;
@@ -141,6 +149,10 @@ int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* c
         int64_t blocks = (waves * kWave + kBlock - 1) / kBlock;
         if (blocks > 8192) blocks = 8192;
         dim3 grid((unsigned)blocks), blk(kBlock);
+        if (timing) {
+            mark();
+            timing->histories += proc;
+        }
         if (fp64) {
             if (use_touched)
                 hipLaunchKernelGGL((k_drop_scatter<true, true>), grid, blk, 0, s, start, proc, off, cnt, vox, chord,
@@ -156,6 +168,7 @@ int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* c
                 hipLaunchKernelGGL((k_drop_scatter<false, false>), grid, blk, 0, s, start, proc, off, cnt, vox, chord,
                                    wepl, x, acc64, acc32, S, touched, n_touched, slot, min_len, lambda);
         }
+        if (timing) mark();
         if (use_touched) {
             int64_t ab = (proc * 64 + kBlock - 1) / kBlock;  // a block touches at most sum(cnt) voxels
             if (ab > 2048) ab = 2048;
@@ -175,40 +188,75 @@ int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* c
                 hipLaunchKernelGGL(k_drop_apply_dense<false>, dim3((unsigned)ab), dim3(kBlock), 0, s, num_voxels, x,
                                    acc64, acc32, S);
         }
+        if (timing) mark();
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         *err = std::string("DROP launch: ") + hipGetErrorString(e);
         return PCT_E_HIP;
     }
+    if (timing) {
+        (void)hipStreamSynchronize(s);
+        for (size_t j = 0; j + 2 < ev.size(); j += 3) {
+            float a = 0, b = 0;
+            (void)hipEventElapsedTime(&a, ev[j], ev[j + 1]);
+            (void)hipEventElapsedTime(&b, ev[j + 1], ev[j + 2]);
+            timing->scatter_ms += a;
+            timing->apply_ms += b;
+            timing->scatter_launches++;
+            timing->apply_launches++;
+        }
+        for (hipEvent_t x_ : ev) (void)hipEventDestroy(x_);
+    }
     return PCT_OK;
 }
 
-int pct_stage_drop(pct_ctx* ctx, int iterations)
+int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing)
+{
+    if (!ctx->plan.ready) PCT_TRY(pct_drop_prepare(ctx));
+    for (int it = 0; it < iterations; it++) PCT_TRY(pct_drop_iterate_plan(ctx, timing));
+    return PCT_OK;
+}
+
+// Algorithmic work of one DROP iteration: histories processed by the blocks, list entries they
+// gather (all of them) and list entries they scatter (histories with >= MIN_MLP_LENGTH voxels).
+int pct_drop_entries(pct_ctx* ctx, int64_t* histories, int64_t* gathered, int64_t* scattered)
 {
     const pct_params& P = ctx->P;
-    const int64_t NV = P.num_voxels;
-    if (!ctx->S) {
-        PCT_TRY(pct_alloc(ctx, &ctx->S, NV));
-        PCT_TRY(pct_alloc(ctx, &ctx->touched, NV));
-        PCT_TRY(pct_alloc(ctx, &ctx->n_touched, 2));
-        PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->S, 0, NV * sizeof(uint32_t), ctx->stream));
+    const int64_t n = ctx->n_recon;
+    std::vector<uint32_t> cnt(n > 0 ? n : 1);
+    if (n > 0) PCT_CHECK_HIP(ctx, hipMemcpy(cnt.data(), ctx->list_cnt, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int64_t g = 0, sc = 0, h = 0;
+    for (int64_t start = 0; start < n; start += P.drop_block_size) {
+        int64_t m = n - start < P.drop_block_size ? n - start : P.drop_block_size;
+        int64_t proc = P.quirk_a3_drop_floor ? (m / P.histories_per_block) * P.histories_per_block : m;
+        for (int64_t i = start; i < start + proc; i++) {
+            g += cnt[i];
+            if ((int64_t)cnt[i] >= P.min_mlp_length) sc += cnt[i];
+        }
+        h += proc;
     }
-    if (P.drop_accum == PCT_ACCUM_FP64 && !ctx->accum64) {
-        PCT_TRY(pct_alloc(ctx, &ctx->accum64, NV));
-        PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->accum64, 0, NV * sizeof(double), ctx->stream));
+    *histories = h;
+    *gathered = g;
+    *scattered = sc;
+    return PCT_OK;
+}
+
+extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* scatter_ms, double* apply_ms,
+                                int64_t* launches, int64_t* histories, int64_t* gathered, int64_t* scattered)
+{
+    if (!ctx || iterations <= 0 || !scatter_ms || !apply_ms || !launches || !histories || !gathered || !scattered)
+        return PCT_E_ARG;
+    if (ctx->stage < ST_LISTS) {
+        ctx->err = "pct_drop_profile: lists not built";
+        return PCT_E_STATE;
     }
-    if (P.drop_accum == PCT_ACCUM_FP32 && !ctx->accum32) {
-        PCT_TRY(pct_alloc(ctx, &ctx->accum32, NV));
-        PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->accum32, 0, NV * sizeof(float), ctx->stream));
-    }
-    for (int it = 0; it < iterations; it++) {
-        int rc = pct_drop_run(ctx->stream, ctx->n_recon, ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord,
-                              ctx->wepl, ctx->x, ctx->accum64, ctx->accum32, ctx->S, ctx->touched, ctx->n_touched, NV,
-                              P.drop_block_size, P.histories_per_block, P.quirk_a3_drop_floor, P.min_mlp_length,
-                              P.lambda, P.drop_accum, &ctx->err);
-        if (rc != PCT_OK) return rc;
-    }
+    DropTiming t;
+    PCT_TRY(pct_stage_drop(ctx, iterations, &t));
+    PCT_TRY(pct_drop_entries(ctx, histories, gathered, scattered));
+    *scatter_ms = t.scatter_ms / (t.scatter_launches ? t.scatter_launches : 1);
+    *apply_ms = t.apply_ms / (t.apply_launches ? t.apply_launches : 1);
+    *launches = t.scatter_launches;
     return PCT_OK;
 }
 

@@ -401,14 +401,17 @@ int pct_stage_build_lists(pct_ctx* ctx)
     a.n_steps = ctx->T.n_steps;
     H10 hp;
     for (int k = 0; k < 10; k++) hp.h[k] = ctx->h[k];
-    PCT_TRY(pct_alloc(ctx, &ctx->list_off, n + 1));
-    PCT_TRY(pct_alloc(ctx, &ctx->list_cnt, n));
-    PCT_TRY(pct_alloc(ctx, &ctx->chord, n));
+    if (n > ctx->list_hist_cap || !ctx->list_off) {  // buffers persist across rebuilds
+        PCT_TRY(pct_alloc(ctx, &ctx->list_off, n + 1));
+        PCT_TRY(pct_alloc(ctx, &ctx->list_cnt, n));
+        PCT_TRY(pct_alloc(ctx, &ctx->chord, n));
+        PCT_TRY(pct_alloc(ctx, &ctx->cap_scratch, n + 1));
+        ctx->list_hist_cap = n;
+    }
     if (!ctx->list_overflow) PCT_TRY(pct_alloc(ctx, &ctx->list_overflow, 1));
     PCT_CHECK_HIP(ctx, hipMemsetAsync(ctx->list_overflow, 0, sizeof(int32_t), s));
     // capacities -> exclusive scan -> offsets
-    int64_t* capv = nullptr;
-    PCT_TRY(pct_alloc(ctx, &capv, n + 1));
+    int64_t* capv = ctx->cap_scratch;
     PCT_CHECK_HIP(ctx, hipMemsetAsync(capv, 0, (n + 1) * sizeof(int64_t), s));
     if (n > 0)
         hipLaunchKernelGGL(k_mlp_capacity, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, hp, ctx->T.sin_t,
@@ -423,7 +426,6 @@ int pct_stage_build_lists(pct_ctx* ctx)
     int64_t total = 0;
     PCT_CHECK_HIP(ctx, hipMemcpyAsync(&total, ctx->list_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    pct_free(&capv);
     if (total > ctx->list_cap) {
         PCT_TRY(pct_alloc(ctx, &ctx->list_vox, total));
         ctx->list_cap = total;
@@ -439,10 +441,21 @@ int pct_stage_build_lists(pct_ctx* ctx)
         int64_t want = (n + kBlock - 1) / kBlock;
         int64_t cap_blocks = (int64_t)dev_cus * 8;
         unsigned blocks = (unsigned)(want < cap_blocks ? want : cap_blocks);
+        hipEvent_t e0, e1;
+        PCT_CHECK_HIP(ctx, hipEventCreate(&e0));
+        PCT_CHECK_HIP(ctx, hipEventCreate(&e1));
+        PCT_CHECK_HIP(ctx, hipEventRecord(e0, s));
         hipLaunchKernelGGL(k_mlp_lists, dim3(blocks), dim3(kBlock), lds, s, n, a, hp, ctx->first_voxel, ctx->T.sin_t,
                            ctx->T.cos_t, ctx->T.scat, ctx->T.p12, ctx->T.p26, ctx->T.p312, ctx->T.step_terms,
                            ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord, ctx->list_overflow);
         PCT_CHECK_HIP(ctx, hipGetLastError());
+        PCT_CHECK_HIP(ctx, hipEventRecord(e1, s));
+        PCT_CHECK_HIP(ctx, hipEventSynchronize(e1));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ctx->t_mlp_kernel = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
     int32_t overflow = 0;
     PCT_CHECK_HIP(ctx, hipMemcpyAsync(&overflow, ctx->list_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -453,5 +466,8 @@ int pct_stage_build_lists(pct_ctx* ctx)
     }
     ctx->list_entries = total;
     ctx->stage = ST_LISTS;
+    // the DROP blocks' transposed lists belong to the list build
+    ctx->plan.ready = false;
+    PCT_TRY(pct_drop_prepare(ctx));
     return PCT_OK;
 }

@@ -247,7 +247,7 @@ extern "C" int pct_params_derive(pct_params* p, char* err, size_t err_len)
     if (p->trig_table_min > -p->pi || p->trig_table_max < 4 * p->pi)
         return fail("trig table must cover [-pi, 4pi] (entry angle, +pi when u_in > u_out)");
     if (p->drop_block_size <= 0 || p->histories_per_block <= 0) return fail("DROP sizes must be positive");
-    if (p->drop_accum != PCT_ACCUM_FP32 && p->drop_accum != PCT_ACCUM_FP64) return fail("DROP_ACCUM must be 0 or 1");
+    if (p->drop_accum < PCT_ACCUM_FP32 || p->drop_accum > PCT_ACCUM_KAHAN32) return fail("DROP_ACCUM must be 0, 1 or 2");
     if (p->hull_type < PCT_HULL_IMPORT || p->hull_type > PCT_HULL_FBP) return fail("HULL_TYPE out of range");
     if (p->x_0_type < PCT_X0_IMPORT || p->x_0_type > PCT_X0_ZEROS) return fail("X_0_TYPE out of range");
     if (p->fbp_filter_type < 0 || p->fbp_filter_type > 2) return fail("FBP_FILTER_TYPE out of range");

@@ -14,7 +14,8 @@ def read_projection(path: str | Path) -> dict:
     reference's data; set DATA_IN_MM to convert)."""
     n = C.c_int64()
     ang = C.c_float()
-    check(lib().pct_read_projection(str(path).encode(), C.byref(n), C.byref(ang), None, None, None, None, None),
+    null4 = (C.c_void_p * 4)()
+    check(lib().pct_read_projection(str(path).encode(), C.byref(n), C.byref(ang), null4, null4, null4, None, None),
           what=f"read {path}")
     n = n.value
     t = np.empty((4, n), np.float32)

@@ -164,6 +164,19 @@ class Reconstruction:
     def synchronize(self) -> None:
         self._call("pct_ctx_synchronize")
 
+    def reset_iterate(self) -> None:
+        """x <- x0 on the device."""
+        self._call("pct_reset_iterate")
+
+    def profile_drop(self, iterations: int = 1) -> dict:
+        """DROP iterations with per-launch HIP-event timing (see pct_drop_profile)."""
+        sc, ap = C.c_double(), C.c_double()
+        la, hi, ga, sa = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        self._call("pct_drop_profile", int(iterations), C.byref(sc), C.byref(ap), C.byref(la), C.byref(hi),
+                   C.byref(ga), C.byref(sa))
+        return {"scatter_ms": sc.value, "apply_ms": ap.value, "launches": la.value, "histories": hi.value,
+                "gathered": ga.value, "scattered": sa.value}
+
     # -- outputs ----------------------------------------------------------------------------------
     @property
     def shape(self) -> tuple[int, int, int]:

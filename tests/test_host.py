@@ -1,0 +1,111 @@
+"""Host-side behaviour of the library that needs no GPU: ABI, parameters, I/O, synthetic data."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pct_reconstruction_amd import _lib, derive, io, load_settings, set_key, synth
+from pct_reconstruction_amd.recon import default_params
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _lib.exported_symbols()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_params_struct_layout_matches_c():
+    assert _lib.lib().pct_params_size() == C.sizeof(_lib.Params)
+
+
+def test_settings_cfg_parsing_and_derivation():
+    p = derive(load_settings(GOLDEN / "settings_head.cfg"))
+    assert (p.columns, p.rows, p.slices) == (256, 256, 96)
+    assert p.recon_cyl_height == pytest.approx(9.61)
+    assert p.voxel_thickness == pytest.approx(9.61 / 96)
+    assert (p.t_bins, p.v_bins, p.angular_bins) == (350, 42, 90)
+    assert p.hull_type == 2 and p.x_0_type == 2 and p.sm_on == 1 and p.sc_on == 0
+    assert p.lambda_ == pytest.approx(1e-4) and p.drop_block_size == 60000
+
+
+def test_settings_errors_are_reported(tmp_path):
+    bad = tmp_path / "bad.cfg"
+    bad.write_text("COLUMNS = 200\nNOT_A_KEY = 3\n")
+    with pytest.raises(ValueError, match="NOT_A_KEY"):
+        load_settings(bad)
+    p = default_params()
+    with pytest.raises(KeyError):
+        set_key(p, "NOPE", 1)
+    with pytest.raises(ValueError):
+        derive(default_params(columns=0))
+    with pytest.raises(ValueError):
+        derive(default_params(recon_cyl_radius=25.0))   # tables cover only 40 cm of depth
+
+
+def test_projection_file_round_trip(tmp_path):
+    ph = synth.phantom_cylinder()
+    d = synth.generate(ph, synth.beam(), 3, 1000)
+    f = tmp_path / "projection_000.bin"
+    io.write_projection(f, d["t"] * 10, d["v"] * 10, d["u"] * 10, d["wepl"] * 10, angle_deg=36.0)
+    r = io.read_projection(f)
+    assert r["header_angle_deg"] == 36.0
+    assert np.array_equal(r["t"], d["t"] * 10) and np.array_equal(r["wepl"], d["wepl"] * 10)
+    assert (r["angle_deg"] == 36.0).all()
+    g = tmp_path / "projection_cont.bin"
+    io.write_projection(g, d["t"], d["v"], d["u"], d["wepl"], angle_deg=0.0, angles=d["angle_deg"])
+    r2 = io.read_projection(g)
+    assert np.array_equal(r2["angle_deg"], d["angle_deg"])
+    (tmp_path / "junk.bin").write_bytes(b"XXXX" + bytes(40))
+    with pytest.raises(RuntimeError):
+        io.read_projection(tmp_path / "junk.bin")
+    with pytest.raises(RuntimeError):
+        io.read_projection(tmp_path / "missing.bin")
+
+
+def test_image_writers(tmp_path):
+    img = (np.arange(2 * 3 * 4, dtype=np.float32) / 7).reshape(2, 3, 4)
+    io.write_image_text(tmp_path / "x.txt", img)
+    lines = (tmp_path / "x.txt").read_text().splitlines()
+    assert len(lines) == 6 and len(lines[0].split()) == 4
+    assert float(lines[5].split()[3]) == pytest.approx(img[1, 2, 3], rel=1e-5)
+    io.write_image_dat(tmp_path / "x.dat", img)
+    assert np.array_equal(np.fromfile(tmp_path / "x.dat", np.float32), (img / 2).reshape(-1))
+
+
+def test_generator_is_deterministic_and_physical():
+    ph = synth.phantom_cylinder()
+    a = synth.generate(ph, synth.beam(), 9, 5000)
+    b = synth.generate(ph, synth.beam(), 9, 5000)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    # noiseless, no scatter: WEPL of a central proton equals the water chord 16 cm
+    bm = synth.beam(wepl_noise_sigma=0.0, outlier_fraction=0.0, scatter=False, t_half_range=1e-6,
+                    v_half_range=1e-6)
+    c = synth.generate(ph, bm, 1, 4)
+    expect = []
+    for ang in np.deg2rad(c["angle_deg"].astype(np.float64)):
+        w = 16.0
+        for phi, rsp in zip(np.deg2rad([0.0, 120.0, 240.0]), (0.3, 1.1, 1.7)):
+            d = 4.0 * abs(np.sin(phi - ang))
+            if d < 1.0:
+                w += (rsp - 1.0) * 2 * np.sqrt(1 - d * d)
+        expect.append(w)
+    assert np.allclose(c["wepl"], expect, atol=1e-4)
+    # integer angles come in angle-major order
+    assert (np.diff(a["angle_deg"]) >= 0).all()
+
+
+def test_rasterize_cylinder_phantom():
+    p = derive(default_params(columns=100, rows=100, slice_thickness=1.0))
+    img = synth.rasterize(synth.phantom_cylinder(), p)
+    assert img.shape == (p.slices, 100, 100)
+    vals = np.unique(img)
+    assert all(np.isclose(v, [0.0, 0.3, 1.0, 1.1, 1.7], atol=1e-6).any() for v in vals)
+    assert img[4, 50, 50] == pytest.approx(1.0)
