@@ -120,20 +120,25 @@ def main() -> int:
     r.reset_iterate()
     prof = r.profile_drop(1)
     mlp_ms = m["kernel_ms"]["mlp_lists"]
-    drop_bytes_iter = 8 * prof["gathered"] + 24 * prof["scattered"] + 24 * prof["histories"]
-    scatter_bytes = drop_bytes_iter / max(prof["launches"], 1)
-    scatter_gbs = scatter_bytes / (prof["scatter_ms"] * 1e-3) / 1e9
+    nb = max(prof["blocks"], 1)
+    # algorithmic bytes per launch: rows = list index + x gather per entry, 32 B of
+    # per-history header/result; cols = history index + r gather per summed entry, voxel id,
+    # offset and x read-modify-write per touched voxel; MLP = 64 B in/out per history + 4 B per entry.
+    rows_bytes = (8 * prof["gathered"] + 32 * prof["histories"]) / nb
+    cols_bytes = (12 * prof["summed"] + 16 * prof["touched"]) / nb
     mlp_bytes = 4 * entries + 64 * n_recon
-    mlp_gbs = mlp_bytes / (mlp_ms * 1e-3) / 1e9
-    drop_total_ms = iters * prof["launches"] * prof["scatter_ms"]
+
+    def kern(ms, nbytes, launches_per_step):
+        return {"avg_launch_ms": ms, "bytes_per_launch": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9,
+                "launches_per_step": launches_per_step, "ms_per_step": ms * launches_per_step}
+
     kernels = {
-        "drop_scatter": {"avg_launch_ms": prof["scatter_ms"], "launches_per_iteration": prof["launches"],
-                         "bytes_per_launch": scatter_bytes, "GBs": scatter_gbs,
-                         "ms_per_step": drop_total_ms},
-        "drop_apply": {"avg_launch_ms": prof["apply_ms"], "ms_per_step": iters * prof["launches"] * prof["apply_ms"]},
-        "mlp_lists": {"avg_launch_ms": mlp_ms, "bytes_per_launch": mlp_bytes, "GBs": mlp_gbs, "ms_per_step": mlp_ms},
+        "drop_rows": kern(prof["rows_ms"], rows_bytes, iters * nb),
+        "drop_cols": kern(prof["cols_ms"], cols_bytes, iters * nb),
+        "mlp_lists": kern(mlp_ms, mlp_bytes, 1),
     }
-    dominant = "drop_scatter" if drop_total_ms >= mlp_ms else "mlp_lists"
+    kernels["list_transpose_ms_per_step"] = max(m["stage_ms"]["mlp_build_lists"] - mlp_ms, 0.0)
+    dominant = max(("drop_rows", "drop_cols", "mlp_lists"), key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
     traffic = None
     tfile = ROOT / "profiles" / f"{dominant}_traffic.json"

@@ -165,12 +165,16 @@ int pct_drop_iterate(pct_ctx* ctx, int iterations);
 int pct_run_pipeline(pct_ctx* ctx);
 /* Reset the iterate x to the initial iterate x0 (on the device). */
 int pct_reset_iterate(pct_ctx* ctx);
+/* Launch shape of the DROP kernels: lanes per history in the row kernel (4, 8, 16, 32, 64) and
+ * voxels per lane in the column kernel (1, 2, 4, 8).  Results do not depend on it. */
+int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_voxels);
 /* Run `iterations` DROP iterations with a HIP event pair around every kernel launch and report the
- * average duration of one block's scatter kernel and apply kernel (ms), the number of blocks
- * launched, and per iteration the histories processed, list entries gathered and list entries
- * scattered.  Updates x like pct_drop_iterate. */
-int pct_drop_profile(pct_ctx* ctx, int iterations, double* scatter_ms, double* apply_ms, int64_t* launches,
-                     int64_t* histories, int64_t* gathered, int64_t* scattered);
+ * average duration of one block's row kernel (per-history updates) and column kernel (per-voxel
+ * sums) in ms, the number of blocks per iteration, and per iteration the histories processed,
+ * list entries gathered, list entries summed per voxel and voxels touched.  Updates x like
+ * pct_drop_iterate. */
+int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, double* cols_ms, int64_t* launches,
+                     int64_t* histories, int64_t* gathered, int64_t* scattered, int64_t* touched);
 
 /* ---------------------------------------------------------------- outputs ---- */
 enum pct_image_kind { PCT_IMG_X = 0, PCT_IMG_FBP = 1, PCT_IMG_X0 = 2, PCT_IMG_HULL = 3,

@@ -122,7 +122,8 @@ def lib() -> C.CDLL:
         "pct_drop_iterate": ([vp, C.c_int], C.c_int),
         "pct_run_pipeline": ([vp], C.c_int),
         "pct_reset_iterate": ([vp], C.c_int),
-        "pct_drop_profile": ([vp, C.c_int, P(dbl), P(dbl), P(i64), P(i64), P(i64), P(i64)], C.c_int),
+        "pct_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
+        "pct_drop_profile": ([vp, C.c_int, P(dbl), P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64)], C.c_int),
         "pct_get_image": ([vp, C.c_int, vp], C.c_int),
         "pct_set_image": ([vp, vp], C.c_int),
         "pct_set_hull": ([vp, vp], C.c_int),

@@ -10,6 +10,7 @@
 //   DROP_GPU                  -> pct_mlp_build_lists + pct_drop_iterate
 // Every stage runs on the context's stream; the host only synchronises to learn compaction counts.
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 
@@ -98,6 +99,15 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
     pct_ctx* ctx = new pct_ctx();
     ctx->P = q;
     ctx->device = device;
+    // launch shapes of the DROP kernels (tuning; the arithmetic does not depend on them)
+    if (const char* e = std::getenv("PCT_DROP_ROW_LANES")) {
+        int g = std::atoi(e);
+        if (g == 4 || g == 8 || g == 16 || g == 32 || g == 64) ctx->drop_row_lanes = g;
+    }
+    if (const char* e = std::getenv("PCT_DROP_COL_VOXELS")) {
+        int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4 || v == 8) ctx->drop_col_voxels = v;
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PCT_E_HIP;
@@ -553,6 +563,16 @@ int pct_convert_units(pct_ctx* ctx, int64_t first, int64_t n)
         return PCT_E_HIP;
     }
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PCT_OK;
+}
+
+extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_voxels)
+{
+    if (!ctx) return PCT_E_ARG;
+    const int g = drop_row_lanes, v = drop_col_voxels;
+    if (!(g == 4 || g == 8 || g == 16 || g == 32 || g == 64) || !(v == 1 || v == 2 || v == 4 || v == 8)) return PCT_E_ARG;
+    ctx->drop_row_lanes = g;
+    ctx->drop_col_voxels = v;
     return PCT_OK;
 }
 

@@ -137,6 +137,8 @@ struct pct_ctx {
 
     // DROP plan (transposed lists) and the scratch of the atomic DROP kernels
     DropPlan plan;
+    int drop_row_lanes = 16;   // lanes per history in the row kernel (PCT_DROP_ROW_LANES: 4..64)
+    int drop_col_voxels = 4;   // voxels per lane in the column kernel (PCT_DROP_COL_VOXELS: 1, 2, 4, 8)
     double* accum64 = nullptr;
     float* accum32 = nullptr;
     uint32_t* S = nullptr;

@@ -242,10 +242,12 @@ int pct_drop_entries(pct_ctx* ctx, int64_t* histories, int64_t* gathered, int64_
     return PCT_OK;
 }
 
-extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* scatter_ms, double* apply_ms,
-                                int64_t* launches, int64_t* histories, int64_t* gathered, int64_t* scattered)
+extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, double* cols_ms,
+                                int64_t* launches, int64_t* histories, int64_t* gathered, int64_t* scattered,
+                                int64_t* touched)
 {
-    if (!ctx || iterations <= 0 || !scatter_ms || !apply_ms || !launches || !histories || !gathered || !scattered)
+    if (!ctx || iterations <= 0 || !rows_ms || !cols_ms || !launches || !histories || !gathered || !scattered ||
+        !touched)
         return PCT_E_ARG;
     if (ctx->stage < ST_LISTS) {
         ctx->err = "pct_drop_profile: lists not built";
@@ -254,9 +256,15 @@ extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* scatter_ms
     DropTiming t;
     PCT_TRY(pct_stage_drop(ctx, iterations, &t));
     PCT_TRY(pct_drop_entries(ctx, histories, gathered, scattered));
-    *scatter_ms = t.scatter_ms / (t.scatter_launches ? t.scatter_launches : 1);
-    *apply_ms = t.apply_ms / (t.apply_launches ? t.apply_launches : 1);
-    *launches = t.scatter_launches;
+    const size_t nb = ctx->plan.blocks.size();
+    std::vector<uint32_t> nv(nb ? nb : 1);
+    if (nb) PCT_CHECK_HIP(ctx, hipMemcpy(nv.data(), ctx->plan.n_vox, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int64_t u = 0;
+    for (size_t b = 0; b < nb; b++) u += nv[b];
+    *touched = u;
+    *rows_ms = t.scatter_ms / (t.scatter_launches ? t.scatter_launches : 1);
+    *cols_ms = t.apply_ms / (t.apply_launches ? t.apply_launches : 1);
+    *launches = t.scatter_launches / iterations;
     return PCT_OK;
 }
 

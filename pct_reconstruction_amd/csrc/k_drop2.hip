@@ -59,7 +59,11 @@ __global__ void k_emit_pairs(int64_t start, int64_t proc, const int64_t* __restr
     }
 }
 
-// per history: u_i (DS.cu:9321-9380 with the list of find_MLP_path_GPU_tabulated)
+// per history: u_i (DS.cu:9321-9380 with the list of find_MLP_path_GPU_tabulated).
+// G lanes per history (G | 64): lane l sums entries l, l+G, l+2G, ... with four gathers in flight,
+// then the G partial sums are combined with a butterfly.  Several histories per wavefront keep
+// enough independent loads in flight to cover the HBM/MALL latency.
+template <int G>
 __global__ void __launch_bounds__(kBlock) k_drop_rows(int64_t start, int64_t proc, const int64_t* __restrict__ off,
                                                        const uint32_t* __restrict__ cnt_arr,
                                                        const uint32_t* __restrict__ vox,
@@ -67,49 +71,96 @@ __global__ void __launch_bounds__(kBlock) k_drop_rows(int64_t start, int64_t pro
                                                        const float* __restrict__ wepl, const float* __restrict__ x,
                                                        double lambda, double* __restrict__ r)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < proc; i += n_waves) {
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t n_groups = ((int64_t)gridDim.x * blockDim.x) / G;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; i < proc; i += n_groups) {
         const int64_t p = start + i;
-        const int64_t base = off[p];
-        const int64_t cnt = cnt_arr[p];
-        const double L = chord[p];
+        const uint32_t* __restrict__ lst = vox + off[p];
+        const int cnt = (int)cnt_arr[p];
         double part = 0.0;
-        for (int64_t j = lane; j < cnt; j += kWave) part += (double)x[vox[base + j]];
-        const double ax = wave_sum(part) * L;
-        const double aa = (double)pct_sqf(L) * (double)cnt;
-        if (lane == 0) r[i] = L * (((double)wepl[p] - ax) / aa) * lambda;
+        int j = lane;
+        for (; j + 3 * G < cnt; j += 4 * G) {
+            const uint32_t a0 = lst[j], a1 = lst[j + G], a2 = lst[j + 2 * G], a3 = lst[j + 3 * G];
+            const float x0 = x[a0], x1 = x[a1], x2 = x[a2], x3 = x[a3];
+            part += (double)x0;
+            part += (double)x1;
+            part += (double)x2;
+            part += (double)x3;
+        }
+        for (; j < cnt; j += G) part += (double)x[lst[j]];
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
+        if (lane == 0) {
+            const double L = chord[p];
+            const double ax = part * L;
+            const double aa = (double)pct_sqf(L) * (double)cnt;
+            r[i] = L * (((double)wepl[p] - ax) / aa) * lambda;
+        }
     }
 }
 
-// per touched voxel: image_update_GPU (DS.cu:9398-9419) with the block's sums formed in history order
 template <int ACC>
-__global__ void k_drop_cols(const uint32_t* __restrict__ t_vox, const uint32_t* __restrict__ t_off,
-                            const uint32_t* __restrict__ t_hist, const uint32_t* __restrict__ n_vox,
-                            const double* __restrict__ r, float* __restrict__ x)
+__device__ __forceinline__ float col_update(uint32_t b, uint32_t e, double first, const uint32_t* __restrict__ t_hist,
+                                            const double* __restrict__ r)
+{
+    if (ACC == PCT_ACCUM_FP64) {
+        double s = first;
+        for (uint32_t k = b + 1; k < e; k++) s += r[t_hist[k]];
+        return (float)(s / (double)(e - b));
+    } else if (ACC == PCT_ACCUM_FP32) {
+        float s = (float)first;
+        for (uint32_t k = b + 1; k < e; k++) s += (float)r[t_hist[k]];
+        return s / (float)(e - b);
+    } else {  // Kahan-compensated FP32
+        float s = 0.0f, c = 0.0f;
+        float y = (float)first - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+        for (uint32_t k = b + 1; k < e; k++) {
+            y = (float)r[t_hist[k]] - c;
+            t = s + y;
+            c = (t - s) - y;
+            s = t;
+        }
+        return s / (float)(e - b);
+    }
+}
+
+// per touched voxel: image_update_GPU (DS.cu:9398-9419) with the block's sums formed in history
+// order.  Each lane takes V voxels (strided by the grid) and issues their loads together before any
+// result is needed: segment bounds, first history, its update and x.
+template <int ACC, int V>
+__global__ void __launch_bounds__(kBlock) k_drop_cols(const uint32_t* __restrict__ t_vox,
+                                                       const uint32_t* __restrict__ t_off,
+                                                       const uint32_t* __restrict__ t_hist,
+                                                       const uint32_t* __restrict__ n_vox,
+                                                       const double* __restrict__ r, float* __restrict__ x)
 {
     const uint32_t U = *n_vox;
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < U; u += gridDim.x * blockDim.x) {
-        const uint32_t b = t_off[u], e = t_off[u + 1];
-        const uint32_t v = t_vox[u];
-        if (ACC == PCT_ACCUM_FP64) {
-            double s = 0.0;
-            for (uint32_t k = b; k < e; k++) s += r[t_hist[k]];
-            x[v] += (float)(s / (double)(e - b));
-        } else if (ACC == PCT_ACCUM_FP32) {
-            float s = 0.0f;
-            for (uint32_t k = b; k < e; k++) s += (float)r[t_hist[k]];
-            x[v] += s / (float)(e - b);
-        } else {  // Kahan-compensated FP32
-            float s = 0.0f, c = 0.0f;
-            for (uint32_t k = b; k < e; k++) {
-                float y = (float)r[t_hist[k]] - c;
-                float t = s + y;
-                c = (t - s) - y;
-                s = t;
-            }
-            x[v] += s / (float)(e - b);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t u0 = blockIdx.x * blockDim.x + threadIdx.x; u0 < U; u0 += stride * V) {
+        uint32_t b[V], e[V], v[V], h0[V];
+        float xv[V];
+        double r0[V];
+#pragma unroll
+        for (int q = 0; q < V; q++) {
+            const uint32_t u = u0 + q * stride;
+            const bool ok = u < U;
+            b[q] = ok ? t_off[u] : 0;
+            e[q] = ok ? t_off[u + 1] : 0;
+            v[q] = ok ? t_vox[u] : 0;
         }
+#pragma unroll
+        for (int q = 0; q < V; q++) h0[q] = e[q] > b[q] ? t_hist[b[q]] : 0;
+#pragma unroll
+        for (int q = 0; q < V; q++) {
+            r0[q] = e[q] > b[q] ? r[h0[q]] : 0.0;
+            xv[q] = e[q] > b[q] ? x[v[q]] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < V; q++)
+            if (e[q] > b[q]) x[v[q]] = xv[q] + col_update<ACC>(b[q], e[q], r0[q], t_hist, r);
     }
 }
 
@@ -246,29 +297,50 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
         (void)hipEventRecord(e, s);
         ev.push_back(e);
     };
+    const int G = ctx->drop_row_lanes, V = ctx->drop_col_voxels;
     for (size_t bi = 0; bi < D.blocks.size(); bi++) {
         const DropPlan::Block& bk = D.blocks[bi];
-        int64_t blocks = (bk.proc * kWave + kBlock - 1) / kBlock;
+        int64_t blocks = (bk.proc * G + kBlock - 1) / kBlock;
         if (blocks > 16384) blocks = 16384;
         if (timing) mark();
-        hipLaunchKernelGGL(k_drop_rows, dim3((unsigned)blocks), dim3(kBlock), 0, s, bk.start, bk.proc, ctx->list_off,
-                           ctx->list_cnt, ctx->list_vox, ctx->chord, ctx->wepl, ctx->x, P.lambda, D.r);
+        dim3 rg((unsigned)blocks), rb(kBlock);
+#define PCT_ROWS(GG)                                                                                             \
+    hipLaunchKernelGGL(k_drop_rows<GG>, rg, rb, 0, s, bk.start, bk.proc, ctx->list_off, ctx->list_cnt, ctx->list_vox, \
+                       ctx->chord, ctx->wepl, ctx->x, P.lambda, D.r)
+        switch (G) {
+        case 4: PCT_ROWS(4); break;
+        case 8: PCT_ROWS(8); break;
+        case 16: PCT_ROWS(16); break;
+        case 32: PCT_ROWS(32); break;
+        default: PCT_ROWS(64); break;
+        }
+#undef PCT_ROWS
         if (timing) mark();
         if (bk.entries > 0) {
-            int64_t cb = (bk.entries + kBlock - 1) / kBlock;
+            int64_t cb = (bk.entries + (int64_t)kBlock * V - 1) / ((int64_t)kBlock * V);
             if (cb > 4096) cb = 4096;
             const uint32_t* t_vox = D.t_vox + bk.entry_base;
             const uint32_t* t_off = D.t_off + bk.entry_base + bi;
             const uint32_t* t_hist = D.t_hist + bk.entry_base;
-            if (P.drop_accum == PCT_ACCUM_FP64)
-                hipLaunchKernelGGL(k_drop_cols<PCT_ACCUM_FP64>, dim3((unsigned)cb), dim3(kBlock), 0, s, t_vox, t_off,
-                                   t_hist, D.n_vox + bi, D.r, ctx->x);
-            else if (P.drop_accum == PCT_ACCUM_FP32)
-                hipLaunchKernelGGL(k_drop_cols<PCT_ACCUM_FP32>, dim3((unsigned)cb), dim3(kBlock), 0, s, t_vox, t_off,
-                                   t_hist, D.n_vox + bi, D.r, ctx->x);
-            else
-                hipLaunchKernelGGL(k_drop_cols<PCT_ACCUM_KAHAN32>, dim3((unsigned)cb), dim3(kBlock), 0, s, t_vox,
-                                   t_off, t_hist, D.n_vox + bi, D.r, ctx->x);
+            dim3 cg((unsigned)cb), cbk(kBlock);
+#define PCT_COLS(A, VV) \
+    hipLaunchKernelGGL((k_drop_cols<A, VV>), cg, cbk, 0, s, t_vox, t_off, t_hist, D.n_vox + bi, D.r, ctx->x)
+#define PCT_COLS_V(A)                  \
+    switch (V) {                       \
+    case 1: PCT_COLS(A, 1); break;     \
+    case 2: PCT_COLS(A, 2); break;     \
+    case 8: PCT_COLS(A, 8); break;     \
+    default: PCT_COLS(A, 4); break;    \
+    }
+            if (P.drop_accum == PCT_ACCUM_FP64) {
+                PCT_COLS_V(PCT_ACCUM_FP64)
+            } else if (P.drop_accum == PCT_ACCUM_FP32) {
+                PCT_COLS_V(PCT_ACCUM_FP32)
+            } else {
+                PCT_COLS_V(PCT_ACCUM_KAHAN32)
+            }
+#undef PCT_COLS_V
+#undef PCT_COLS
         }
         if (timing) {
             mark();

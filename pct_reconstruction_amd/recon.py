@@ -164,18 +164,22 @@ class Reconstruction:
     def synchronize(self) -> None:
         self._call("pct_ctx_synchronize")
 
+    def set_launch(self, drop_row_lanes: int, drop_col_voxels: int) -> None:
+        """Launch shape of the DROP kernels (see pct_set_launch)."""
+        self._call("pct_set_launch", int(drop_row_lanes), int(drop_col_voxels))
+
     def reset_iterate(self) -> None:
         """x <- x0 on the device."""
         self._call("pct_reset_iterate")
 
     def profile_drop(self, iterations: int = 1) -> dict:
         """DROP iterations with per-launch HIP-event timing (see pct_drop_profile)."""
-        sc, ap = C.c_double(), C.c_double()
-        la, hi, ga, sa = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
-        self._call("pct_drop_profile", int(iterations), C.byref(sc), C.byref(ap), C.byref(la), C.byref(hi),
-                   C.byref(ga), C.byref(sa))
-        return {"scatter_ms": sc.value, "apply_ms": ap.value, "launches": la.value, "histories": hi.value,
-                "gathered": ga.value, "scattered": sa.value}
+        ro, co = C.c_double(), C.c_double()
+        la, hi, ga, sa, tu = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        self._call("pct_drop_profile", int(iterations), C.byref(ro), C.byref(co), C.byref(la), C.byref(hi),
+                   C.byref(ga), C.byref(sa), C.byref(tu))
+        return {"rows_ms": ro.value, "cols_ms": co.value, "blocks": la.value, "histories": hi.value,
+                "gathered": ga.value, "summed": sa.value, "touched": tu.value}
 
     # -- outputs ----------------------------------------------------------------------------------
     @property
