@@ -239,6 +239,101 @@ def test_msc_edge_tiny_brute_force(oracle):
             assert out[0, r, c] == exp, (r, c)
 
 
+def test_msc_sm_counts_follow_the_sc_walk_and_gates(oracle):
+    """MSC/SM add one count on exactly the voxels SC carves, for WEPL inside their windows only."""
+    P = _params(columns=40, rows=40, slice_thickness=1.0, msc_lower_threshold=0.0, msc_upper_threshold=2.0,
+                sm_lower_threshold=5.0)
+    h = np.zeros((10, 1), np.float32)
+    h[0], h[1], h[2] = -9.5, -2.2, 0.3
+    h[3], h[4], h[5] = 9.3, 1.7, -0.8
+    init8 = oracle.init_hull(P, np.uint8)
+    path = (init8 == 1) & (oracle.carve(P, "sc", h, np.array([-0.5], np.float32)) == 0)
+    init = oracle.init_hull(P, np.int32)
+    for method, wepl, inside in [("msc", 1.0, True), ("msc", 2.5, False), ("msc", -0.1, False),
+                                 ("sm", 5.0, True), ("sm", 4.9, False)]:
+        counts = oracle.carve(P, method, h, np.array([wepl], np.float32))
+        exp = init + (path.astype(np.int32) if inside else 0)
+        assert (counts == exp).all(), (method, wepl)
+
+
+def test_sm_edge_threshold_is_count_at_first_argmax(oracle):
+    """Per slice, threshold = count at the first (row-major) pixel of largest 3x3 difference."""
+    P = _params(columns=16, rows=16, slice_thickness=8.0, recon_cyl_radius=10.0, sm_scale_threshold=1.0)
+    counts = np.full((P.slices, 16, 16), 10, np.int32)
+    counts[0, 4:7, 4:7] = 40          # first pixel of difference 30 (row 4): threshold 40
+    counts[0, 8:13, 8:13] = 20
+    counts[0, 9:12, 9:12] = 50        # same difference 30, later in the scan
+    out = oracle.sm_edge(P, counts.reshape(-1)).reshape(counts.shape)
+    exp = np.zeros_like(out)
+    exp[0, 9:12, 9:12] = 1            # counts > 1.0 * 40
+    assert (out == exp).all()
+    # a larger scale keeps nothing; a smaller one keeps both squares and the ring above 0.45*40
+    P2 = _params(columns=16, rows=16, slice_thickness=8.0, recon_cyl_radius=10.0, sm_scale_threshold=0.45)
+    out2 = oracle.sm_edge(P2, counts.reshape(-1)).reshape(counts.shape)
+    assert (out2[0] == (counts[0] > 18)).all()
+
+
+def test_fbp_hull_threshold_and_cylinder(oracle):
+    P = _params(columns=20, rows=20, slice_thickness=4.0, recon_cyl_radius=10.0, fbp_threshold=0.6)
+    col = np.arange(20)
+    # float image against the double threshold (C promotion): float(0.6) = 0.6000000238 > 0.6 is kept,
+    # the float just below it is not
+    below = np.nextafter(np.float32(0.6), np.float32(0))
+    img = np.where(col % 3 == 0, np.float32(0.6), np.where(col % 3 == 1, below, np.float32(0.7))).astype(np.float32)
+    img = np.broadcast_to(img[None, None, :], (P.slices, 20, 20)).copy()
+    out = oracle.fbp_hull(P, img.reshape(-1)).reshape(img.shape)
+    x = -10 + (col + 0.5)
+    y = 10 - (np.arange(20) + 0.5)
+    inside = (x[None, :] ** 2 + y[:, None] ** 2) < 100
+    exp = (col[None, :] % 3 != 1) & inside      # above the threshold, centre strictly inside
+    assert (out == exp[None]).all()
+
+
+def test_construct_sinogram_divides_non_empty_bins(oracle):
+    P = _params(columns=20, rows=20)
+    s = np.arange(P.num_bins, dtype=np.float32) * 0.5
+    cnt = np.zeros(P.num_bins, np.int32)
+    cnt[::7] = 4
+    out = oracle.construct_sinogram(P, cnt, s)
+    exp = s.copy()
+    exp[::7] = s[::7] / np.float32(4)
+    assert np.array_equal(out, exp)
+
+
+def test_endpoints_on_the_cylinder_hull(oracle):
+    """Straight protons along x: the entry/exit walk stops at the first/last voxel of the row whose
+    centre lies inside the cylinder hull; a proton missing the hull gets first voxel NUM_VOXELS."""
+    P = _params(columns=40, rows=40, slice_thickness=1.0, recon_cyl_radius=10.0)
+    hull = oracle.init_hull(P, np.uint8)
+    hv = hull.reshape(P.slices, P.rows, P.columns)
+    ys = np.array([0.3, -4.1, 7.7, 9.2], np.float64)
+    zs = np.array([0.1, -2.6, 3.3, 0.7], np.float64)
+    n = ys.size
+    h = np.zeros((10, n), np.float32)
+    xe = -np.sqrt(100.0 - ys ** 2)
+    h[0], h[1], h[2] = xe, ys, zs
+    h[3], h[4], h[5] = -xe, ys, zs
+    h_new, first = oracle.endpoints(P, hull, h)
+    for i in range(n):
+        r = int((10 - ys[i]) / 0.5)
+        s = int((P.recon_cyl_height / 2 - zs[i]) / 1.0)
+        cols = np.flatnonzero(hv[s, r])
+        c_in, c_out = cols[0], cols[-1]
+        assert first[i] == c_in + r * 40 + s * 1600, i
+        # a walk that starts in a hull voxel keeps its point; otherwise the point moves to the face
+        # of the hit voxel (left face on entry, right face on exit)
+        c0 = int((h[0, i] + 10) / 0.5)
+        c1 = int((h[3, i] + 10) / 0.5)
+        exp_in = h[0, i] if c0 == c_in else -10 + c_in * 0.5
+        exp_out = h[3, i] if c1 == c_out else -10 + (c_out + 1) * 0.5
+        assert abs(h_new[0, i] - exp_in) < 1e-5, i
+        assert abs(h_new[3, i] - exp_out) < 1e-5, i
+        assert h_new[1, i] == h[1, i] and h_new[2, i] == h[2, i]
+    assert (first != P.num_voxels).all()
+    _, first0 = oracle.endpoints(P, np.zeros_like(hull), h)
+    assert (first0 == P.num_voxels).all()
+
+
 # ---------------------------------------------------------------- FBP
 def test_backprojection_of_constant_sinogram_at_centre(oracle):
     P = _params(columns=41, rows=41, slice_thickness=8.0, recon_cyl_radius=10.0)
