@@ -1,6 +1,6 @@
 """MLP+DROP throughput on one MI355X: the head-50M workload.
 
-Workload (BASELINE configuration 2): 10-ellipsoid head phantom, 50M seeded synthetic histories with
+Workload: 10-ellipsoid head phantom, 50M seeded synthetic histories with
 continuous gantry angles, 256x256x96 grid at ~1 mm, MSC + SM hulls, 3-sigma cuts, FBP x0, MLP + DROP
 for 20 iterations with block 60000.  Preprocessing (binning, cuts, hulls, FBP, endpoints) runs once
 before the timed region; one timed step = building the MLP lists of all reconstruction histories
@@ -31,9 +31,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(P, r, sample: int, threads: int, iterations: int) -> dict:
-    """Oracle (C, OpenMP) MLP lists + one DROP iteration on the first `sample` reconstruction
-    histories, extrapolated per history-iteration like the GPU metric."""
+def cpu_baseline(P, r, sample: int, threads: int, iterations: int, cpu_iterations: int) -> dict:
+    """Oracle (C, OpenMP) MLP lists + `cpu_iterations` DROP iterations on the first `sample`
+    reconstruction histories, extrapolated per history-iteration like the GPU metric."""
     sys.path.insert(0, str(ROOT / "oracle"))
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     import pct_oracle as O
@@ -47,14 +47,17 @@ def cpu_baseline(P, r, sample: int, threads: int, iterations: int) -> dict:
     t0 = time.perf_counter()
     off, vox, chord = O.mlp_lists(P, tabs, h, first)
     t_mlp = time.perf_counter() - t0
+    x = x0.copy()
     t0 = time.perf_counter()
-    O.drop_iteration(P, off, vox, chord, w, x0, accum=0, nthreads=threads)
-    t_drop = time.perf_counter() - t0
+    for _ in range(cpu_iterations):
+        x = O.drop_iteration(P, off, vox, chord, w, x, accum=0, nthreads=threads)
+    t_drop = (time.perf_counter() - t0) / cpu_iterations
     n = h.shape[1]
     rate = n * iterations / (t_mlp + iterations * t_drop)
     return {"value": rate, "unit": "histories/s", "cores": threads, "kind": "port",
             "sample": f"first {n} reconstruction histories of the workload: oracle MLP lists ({t_mlp:.2f} s) + "
-                      f"one DROP iteration ({t_drop:.2f} s), rate = n*{iterations}/(t_mlp + {iterations}*t_drop)"}
+                      f"{cpu_iterations} DROP iterations ({t_drop:.2f} s each, FP32 reference-precision sums), "
+                      f"rate = n*{iterations}/(t_mlp + {iterations}*t_drop)"}
 
 
 def main() -> int:
@@ -64,13 +67,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=int, default=50_000_000)
     ap.add_argument("--iterations", type=int, default=None, help="DROP iterations per step (default: config)")
-    ap.add_argument("--cpu-sample", type=int, default=120_000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="reconstruction histories timed on the CPU (0: all of them)")
+    ap.add_argument("--cpu-iterations", type=int, default=2, help="DROP iterations timed on the CPU")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not implemented)"}))
+        print(json.dumps({"error": "runs on one GPU only: --gpus must be 1 (sharding over GPUs is a design, "
+                                   "not implemented)"}))
         return 2
 
     from pct_reconstruction_amd import Reconstruction, configs, synth
@@ -121,7 +126,7 @@ def main() -> int:
     prof = r.profile_drop(1)
     mlp_ms = m["kernel_ms"]["mlp_lists"]
     nb = max(prof["blocks"], 1)
-    # algorithmic bytes per launch: rows = list index + x gather per entry, 32 B of
+    # algorithmic bytes per launch of this implementation: rows = list index + x gather per entry, 32 B of
     # per-history header/result; cols = history index + r gather per summed entry, voxel id,
     # offset and x read-modify-write per touched voxel; MLP = 64 B in/out per history + 4 B per entry.
     rows_bytes = (8 * prof["gathered"] + 32 * prof["histories"]) / nb
@@ -138,23 +143,34 @@ def main() -> int:
         "mlp_lists": kern(mlp_ms, mlp_bytes, 1),
     }
     kernels["list_transpose_ms_per_step"] = max(m["stage_ms"]["mlp_build_lists"] - mlp_ms, 0.0)
-    dominant = max(("drop_rows", "drop_cols", "mlp_lists"), key=lambda k: kernels[k]["ms_per_step"])
-    dk = kernels[dominant]
+    # The dominant work is the DROP block update, which the reference performs as
+    # calculate_x_update_GPU + image_update_GPU and this build as the pair k_drop_rows + k_drop_cols.
+    # Its algorithmic bytes per proton-iteration are B_alg = 18 n + 12 (list entry 6 B incl. chord,
+    # x gather 4 B, x_update read-modify-write 8 B per voxel; 8 B header + 4 B WEPL per proton),
+    # priced per block and divided by the block's average rows + cols duration.
+    block_ms = prof["rows_ms"] + prof["cols_ms"]
+    block_bytes = (18 * prof["gathered"] + 12 * prof["histories"]) / nb
+    kernels["drop_block"] = kern(block_ms, block_bytes, iters * nb)
     traffic = None
-    tfile = ROOT / "profiles" / f"{dominant}_traffic.json"
+    tfile = ROOT / "profiles" / "drop_block_traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
+            t = json.loads(tfile.read_text())
+            if t.get("workload_histories") == n:
+                traffic = t.get("hbm_bytes_per_block")
+        except (OSError, ValueError):
             traffic = None
-    roofline = {"bound": "hbm", "kernel": dominant, "achieved": dk["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": dk["GBs"] / HBM_PEAK_GBS, "traffic": traffic}
+    roofline = {"bound": "hbm", "kernel": "drop_block (k_drop_rows + k_drop_cols per DROP block)",
+                "achieved": block_bytes / (block_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": block_bytes / (block_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                "bytes_per_launch": block_bytes, "avg_launch_ms": block_ms}
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"cpu baseline on {args.cpu_sample} histories, {threads} threads")
-        cpu = cpu_baseline(r.params, r, min(args.cpu_sample, n_recon), threads, iters)
+        sample = n_recon if args.cpu_sample <= 0 else min(args.cpu_sample, n_recon)
+        log(f"cpu baseline on {sample} histories, {threads} threads")
+        cpu = cpu_baseline(r.params, r, sample, threads, iters, max(args.cpu_iterations, 1))
 
     stages = {k: round(v, 3) for k, v in m["stage_ms"].items()}
     out = {

@@ -39,6 +39,11 @@ enum pct_x0_type { PCT_X0_IMPORT = 0, PCT_X0_HULL = 1, PCT_X0_FBP = 2, PCT_X0_HY
 /* DROP per-voxel accumulation precision (the reference uses float atomicAdd, DS.cu:9533):
  * FP32 sums in history order (reference precision), FP64 sums, or FP32 with Kahan compensation. */
 enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1, PCT_ACCUM_KAHAN32 = 2 };
+/* DROP scheduling flags (pct_set_launch).  Neither changes any result bit:
+ * SPATIAL_ORDER visits a block's histories sorted by sinogram bin (v, angle, t) so that a wavefront
+ * gathers along neighbouring parallel paths; XCD_SLABS gives each of the 8 XCDs a contiguous slab
+ * of those histories (row kernel) and of the touched voxels (column kernel). */
+enum pct_drop_flags { PCT_DROP_SPATIAL_ORDER = 1, PCT_DROP_XCD_SLABS = 2 };
 
 /*
  * Parameters.  Field names follow the settings.cfg keys (settings.cfg:1-164) and the
@@ -165,9 +170,10 @@ int pct_drop_iterate(pct_ctx* ctx, int iterations);
 int pct_run_pipeline(pct_ctx* ctx);
 /* Reset the iterate x to the initial iterate x0 (on the device). */
 int pct_reset_iterate(pct_ctx* ctx);
-/* Launch shape of the DROP kernels: lanes per history in the row kernel (4, 8, 16, 32, 64) and
- * voxels per lane in the column kernel (1, 2, 4, 8).  Results do not depend on it. */
-int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_voxels);
+/* Launch shape of the DROP kernels: lanes per history in the row kernel (4, 8, 16, 32, 64), voxels
+ * per lane in the column kernel (1, 2, 4, 8) and enum pct_drop_flags (a change of flags rebuilds
+ * the transposed lists at the next DROP call).  Results do not depend on any of them. */
+int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_voxels, int drop_flags);
 /* Run `iterations` DROP iterations with a HIP event pair around every kernel launch and report the
  * average duration of one block's row kernel (per-history updates) and column kernel (per-voxel
  * sums) in ms, the number of blocks per iteration, and per iteration the histories processed,
@@ -262,6 +268,36 @@ int pct_generate_histories(const pct_phantom* ph, const pct_beam* beam, uint64_t
                            float* const v[4], float* const u[4], float* wepl, float* angle_deg);
 /* Rasterise the phantom's RSP on the reconstruction grid of *p (voxel centres, NUM_VOXELS floats). */
 int pct_phantom_rasterize(const pct_phantom* ph, const pct_params* p, float* out);
+
+/*
+ * Learned warm start (no reference counterpart: the reference's initial iterate is FBP, the hull or
+ * their product, define_initial_iterate DS.cu:8612-8646).  A 4-level 3D U-Net (16-32-64-128
+ * channels, 3x3x3 convolutions, BF16 weights and activations, FP32 accumulation on MFMA) maps
+ * (FBP image, hull) to x0 = hull * (FBP + correction).  Grid dimensions must be multiples of 8.
+ * Parameters are one FP32 vector: per convolution W[Cout][27][Cin] then b[Cout] for enc1..enc4,
+ * dec3, dec2, dec1 (Cin 8/16/32/64/192/96/48, Cout 16/32/64/128/64/32/16), then the head w[16], b.
+ */
+typedef struct pct_net pct_net;
+int pct_net_create(int columns, int rows, int slices, uint64_t seed, int device, pct_net** out);
+int pct_net_destroy(pct_net* net);
+const char* pct_net_last_error(pct_net* net);
+int64_t pct_net_num_params(pct_net* net);
+int pct_net_get_params(pct_net* net, float* out, int64_t n);
+int pct_net_set_params(pct_net* net, const float* in, int64_t n);
+/* Masked MSE loss over hull voxels of x0 against `target`, and its gradient (grad may be NULL). */
+int pct_net_loss_grad(pct_net* net, const float* fbp, const uint8_t* hull, const float* target, double* loss,
+                      float* grad);
+int pct_net_predict(pct_net* net, const float* fbp, const uint8_t* hull, float* x0);
+/* Training set held in HBM: (FBP, hull, converged DROP image) triples. */
+int pct_net_add_sample(pct_net* net, const float* fbp, const uint8_t* hull, const float* target);
+int pct_net_num_samples(pct_net* net);
+/* `steps` Adam steps (beta 0.9/0.999), one sample per step in a seeded shuffled order. */
+int pct_net_train(pct_net* net, int steps, double lr, uint64_t seed, double* mean_loss);
+int pct_net_save(pct_net* net, const char* path);
+int pct_net_load(pct_net* net, const char* path);
+/* Replace the initial iterate of `ctx` (after pct_mlp_build_lists) by the net's x0 when its DROP data
+ * residual ||Ax - b|| is below that of the current x0; x is reset to the chosen x0. */
+int pct_net_apply(pct_net* net, pct_ctx* ctx, int* used_net, double* residual_fbp, double* residual_net);
 
 #ifdef __cplusplus
 }

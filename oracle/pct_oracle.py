@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
         "o_drop_iteration": ([P(Params), i64, vp, vp, vp, vp, vp, C.c_int, C.c_int], None),
         "o_drop_iteration_fp64": ([P(Params), i64, vp, vp, vp, vp, vp], None),
         "o_residual": ([i64, vp, vp, vp, vp, vp, C.c_int], C.c_double),
+        "on_num_params": ([], i64),
+        "on_loss_grad": ([C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp], C.c_double),
     }
     for name, (a, r) in sig.items():
         fn = getattr(L, name)
@@ -332,3 +334,23 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     if with_fp64:
         out["x_fp64"] = x64
     return out
+
+
+# ------------------------------------------------------------------ learned warm-start network
+def net_num_params() -> int:
+    return int(lib().on_num_params())
+
+
+def net_loss_grad(shape, params, fbp, hull, target, bf16=False, with_grad=True):
+    """FP64 loss, gradient and x0 of the warm-start network (pct_net_oracle.c); shape = (D, H, W).
+    bf16=True rounds to bfloat16 where the GPU stores bfloat16."""
+    D, H, W = shape
+    p = np.ascontiguousarray(params, np.float32)
+    f = np.ascontiguousarray(fbp, np.float32).reshape(-1)
+    h = np.ascontiguousarray(hull, np.uint8).reshape(-1)
+    t = np.ascontiguousarray(target, np.float32).reshape(-1)
+    g = np.zeros(p.size, np.float64) if with_grad else None
+    x0 = np.zeros(f.size, np.float64)
+    loss = lib().on_loss_grad(D, H, W, _p(p), _p(f), _p(h), _p(t), int(bool(bf16)), _p(g) if g is not None else None,
+                              _p(x0))
+    return loss, g, x0.reshape(shape)

@@ -20,7 +20,8 @@ ERRORS = {-1: "invalid argument", -2: "stage called out of order", -3: "HIP runt
 HULL_IMPORT, HULL_SC, HULL_MSC, HULL_SM, HULL_FBP = 0, 1, 2, 3, 4
 FILTER_RAM_LAK, FILTER_SHEPP_LOGAN, FILTER_NONE = 0, 1, 2
 X0_IMPORT, X0_HULL, X0_FBP, X0_HYBRID, X0_ZEROS = 0, 1, 2, 3, 4
-ACCUM_FP32, ACCUM_FP64 = 0, 1
+ACCUM_FP32, ACCUM_FP64, ACCUM_KAHAN32 = 0, 1, 2
+DROP_SPATIAL_ORDER, DROP_XCD_SLABS = 1, 2
 
 IMG = {"x": 0, "fbp": 1, "x0": 2, "hull": 3, "sc_hull": 4, "msc_counts": 5, "sm_counts": 6, "msc_hull": 7,
        "sm_hull": 8, "fbp_hull": 9}
@@ -122,7 +123,7 @@ def lib() -> C.CDLL:
         "pct_drop_iterate": ([vp, C.c_int], C.c_int),
         "pct_run_pipeline": ([vp], C.c_int),
         "pct_reset_iterate": ([vp], C.c_int),
-        "pct_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
+        "pct_set_launch": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
         "pct_drop_profile": ([vp, C.c_int, P(dbl), P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64)], C.c_int),
         "pct_get_image": ([vp, C.c_int, vp], C.c_int),
         "pct_set_image": ([vp, vp], C.c_int),
@@ -143,6 +144,20 @@ def lib() -> C.CDLL:
         "pct_phantom_random": ([P(Phantom), C.c_uint64, C.c_int, dbl, dbl], None),
         "pct_generate_histories": ([P(Phantom), P(Beam), C.c_uint64, i64, fptr4, fptr4, fptr4, vp, vp], C.c_int),
         "pct_phantom_rasterize": ([P(Phantom), P(Params), vp], C.c_int),
+        "pct_net_create": ([C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, P(vp)], C.c_int),
+        "pct_net_destroy": ([vp], C.c_int),
+        "pct_net_last_error": ([vp], C.c_char_p),
+        "pct_net_num_params": ([vp], i64),
+        "pct_net_get_params": ([vp, vp, i64], C.c_int),
+        "pct_net_set_params": ([vp, vp, i64], C.c_int),
+        "pct_net_loss_grad": ([vp, vp, vp, vp, P(dbl), vp], C.c_int),
+        "pct_net_predict": ([vp, vp, vp, vp], C.c_int),
+        "pct_net_add_sample": ([vp, vp, vp, vp], C.c_int),
+        "pct_net_num_samples": ([vp], C.c_int),
+        "pct_net_train": ([vp, C.c_int, dbl, C.c_uint64, P(dbl)], C.c_int),
+        "pct_net_save": ([vp, C.c_char_p], C.c_int),
+        "pct_net_load": ([vp, C.c_char_p], C.c_int),
+        "pct_net_apply": ([vp, vp, P(C.c_int), P(dbl), P(dbl)], C.c_int),
     }
     for name, (argtypes, restype) in sigs.items():
         fn = getattr(L, name)

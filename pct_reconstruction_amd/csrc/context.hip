@@ -108,6 +108,10 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
         int v = std::atoi(e);
         if (v == 1 || v == 2 || v == 4 || v == 8) ctx->drop_col_voxels = v;
     }
+    if (const char* e = std::getenv("PCT_DROP_FLAGS")) {
+        int f = std::atoi(e);
+        if (f >= 0 && f <= 3) ctx->drop_flags = f;
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PCT_E_HIP;
@@ -146,7 +150,7 @@ int pct_ctx_destroy(pct_ctx* ctx)
     pct_free(&ctx->accum64); pct_free(&ctx->accum32); pct_free(&ctx->S); pct_free(&ctx->touched);
     pct_free(&ctx->n_touched); pct_free(&ctx->list_overflow);
     pct_free(&ctx->plan.t_vox); pct_free(&ctx->plan.t_off); pct_free(&ctx->plan.t_hist); pct_free(&ctx->plan.n_vox);
-    pct_free(&ctx->plan.r);
+    pct_free(&ctx->plan.r); pct_free(&ctx->plan.perm); pct_free(&ctx->plan.desc);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PCT_OK;
@@ -566,13 +570,15 @@ int pct_convert_units(pct_ctx* ctx, int64_t first, int64_t n)
     return PCT_OK;
 }
 
-extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_voxels)
+extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_voxels, int drop_flags)
 {
     if (!ctx) return PCT_E_ARG;
     const int g = drop_row_lanes, v = drop_col_voxels;
     if (!(g == 4 || g == 8 || g == 16 || g == 32 || g == 64) || !(v == 1 || v == 2 || v == 4 || v == 8)) return PCT_E_ARG;
+    if (drop_flags & ~(PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS)) return PCT_E_ARG;
     ctx->drop_row_lanes = g;
     ctx->drop_col_voxels = v;
+    ctx->drop_flags = drop_flags;
     return PCT_OK;
 }
 

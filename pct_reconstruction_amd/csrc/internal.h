@@ -50,22 +50,35 @@ struct DevTables {
     int64_t n_trig = 0, n_coeff = 0, n_poly = 0;
 };
 
+// Per history, in visiting order: what the row kernel needs before its list (one 24-byte load).
+struct RowDesc {
+    int64_t off;   // first list entry
+    double chord;  // effective chord length
+    uint32_t cnt;  // list length
+    float wepl;
+};
+
 // Per-block transposed lists for the atomic-free DROP (k_drop2.hip).  Block b covers histories
 // [start, start + proc); its `entries` list entries (histories with >= MIN_MLP_LENGTH voxels) are
-// grouped by voxel: t_vox[u] touched voxel, t_off[u..u+1] its range in t_hist (history index
-// within the block, ascending).  Arrays of all blocks are concatenated at entry_base
-// (t_off at entry_base + b, one extra element per block).
+// grouped by voxel: t_vox[u] touched voxel, t_off[u..u+1] its range in t_hist (visiting rank of
+// each history through the voxel, in ascending history order).  Arrays of all blocks are
+// concatenated at entry_base (t_off at entry_base + b, one extra element per block).
+
 struct DropPlan {
     struct Block {
         int64_t start, proc, entry_base, entries;
     };
     std::vector<Block> blocks;
+    std::vector<uint32_t> n_vox_host;  // per block: number of touched voxels (host copy)
     uint32_t* t_vox = nullptr;
     uint32_t* t_off = nullptr;
-    uint32_t* t_hist = nullptr;
-    uint32_t* n_vox = nullptr;  // per block: number of touched voxels
-    double* r = nullptr;        // per history of the current block: its update value
-    int64_t cap_entries = 0, cap_off = 0, cap_blocks = 0, cap_r = 0;
+    uint32_t* t_hist = nullptr;  // per entry: the visiting rank of the history (ascending history order)
+    uint32_t* n_vox = nullptr;   // per block: number of touched voxels
+    uint32_t* perm = nullptr;    // per reconstruction history: in-block index of the history visited at rank k
+    RowDesc* desc = nullptr;     // per reconstruction history, in visiting order
+    double* r = nullptr;         // per rank of the current block: the update value of that history
+    int64_t cap_entries = 0, cap_off = 0, cap_blocks = 0, cap_r = 0, cap_perm = 0;
+    int flags = -1;              // PCT_DROP_* flags the plan was built with
     bool ready = false;
 };
 
@@ -139,6 +152,7 @@ struct pct_ctx {
     DropPlan plan;
     int drop_row_lanes = 16;   // lanes per history in the row kernel (PCT_DROP_ROW_LANES: 4..64)
     int drop_col_voxels = 4;   // voxels per lane in the column kernel (PCT_DROP_COL_VOXELS: 1, 2, 4, 8)
+    int drop_flags = PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS;  // PCT_DROP_FLAGS
     double* accum64 = nullptr;
     float* accum32 = nullptr;
     uint32_t* S = nullptr;

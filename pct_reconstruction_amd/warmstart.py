@@ -1,0 +1,155 @@
+"""Learned warm start, end to end: training pairs from the pipeline, GPU training, held-out evaluation.
+
+    python -m pct_reconstruction_amd.warmstart --train 400 --test 20 --histories 10000000 \
+        --weights warmstart.bin --report warmstart.json
+
+Workload (learned-128): random ellipsoid phantoms (6-14 ellipsoids, RSP 0-1.8) on 128x128x64 at
+2 mm, 10M seeded histories each, MSC hull, FBP x0.  Training pair = (FBP image, hull, DROP image
+after 40 iterations from the FBP start).  Evaluation per held-out seed:
+  FBP start     x0 = FBP, `ref_iterations` DROP iterations -> residual r_ref = ||Ax - b||, time t_fbp;
+  learned start x0 = net(FBP, hull) if it lowers ||Ax - b|| (else FBP), then DROP iterations until the
+                residual is <= r_ref (at least one) -> iterations k, time t_net (net inference and the
+                residual judgement included).
+Reported: k, t_fbp / t_net, the relative L2 difference between the two final images.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import configs, synth
+from .learned import WarmStartNet
+from .recon import Reconstruction
+
+
+def _prepare(seed: int, n: int):
+    P, ph, bm, n = configs.learned_128(seed, n)
+    d = synth.generate(ph, bm, seed, n)
+    r = Reconstruction(P)
+    r.load_histories(d["t"], d["v"], d["u"], d["wepl"], d["angle_deg"])
+    del d
+    r.preprocessing()
+    r.reconstruction_cuts()
+    r.build_mlp_lists()
+    r.synchronize()
+    return P, r
+
+
+def training_pair(seed: int, n: int, iterations: int = 40):
+    """(FBP, hull, DROP image after `iterations` from the FBP start) of phantom `seed`."""
+    P, r = _prepare(seed, n)
+    try:
+        fbp = r.image("fbp")
+        hull = r.image("hull").astype(np.uint8)
+        r.DROP(iterations)
+        target = r.image("x")
+    finally:
+        r.close()
+    return fbp, hull, target
+
+
+def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int) -> dict:
+    P, r = _prepare(seed, n)
+    try:
+        # FBP start
+        r.reset_iterate()
+        r.synchronize()
+        t0 = time.perf_counter()
+        r.DROP(ref_iterations)
+        r.synchronize()
+        t_fbp = time.perf_counter() - t0
+        r_ref = r.residual()
+        x_fbp = r.image("x")
+        r.reset_iterate()
+        r0_fbp = r.residual()
+        # learned start: count the iterations needed (residual checked after each)
+        judge = net.apply(r)
+        k = 0
+        res = judge["residual_net"] if judge["used_net"] else judge["residual_fbp"]
+        while k < 1 or (res > r_ref and k < 4 * ref_iterations):
+            r.DROP(1)
+            k += 1
+            res = r.residual()
+        x_net = r.image("x")
+        # timed re-run: inference + residual judgement + k iterations
+        r.reset_iterate()
+        r.synchronize()
+        t0 = time.perf_counter()
+        net.apply(r)
+        r.DROP(k)
+        r.synchronize()
+        t_net = time.perf_counter() - t0
+    finally:
+        r.close()
+    return {"seed": seed, "used_net": judge["used_net"], "residual_x0_fbp": r0_fbp,
+            "residual_x0_net": judge["residual_net"], "residual_target": r_ref, "residual_reached": res,
+            "iterations_fbp": ref_iterations, "iterations_net": k, "t_fbp_s": t_fbp, "t_net_s": t_net,
+            "speedup": t_fbp / t_net if t_net > 0 else None,
+            "image_rel_diff": float(np.linalg.norm(x_net - x_fbp) / np.linalg.norm(x_fbp))}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=400, help="training phantoms (seeds 0..train-1)")
+    ap.add_argument("--test", type=int, default=20, help="held-out phantoms (seeds 100000..)")
+    ap.add_argument("--histories", type=int, default=10_000_000)
+    ap.add_argument("--target-iterations", type=int, default=40)
+    ap.add_argument("--ref-iterations", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=4000, help="Adam steps (one phantom each)")
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--weights", default=None, help="write (or with --eval-only read) the weights here")
+    ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--report", default=None)
+    args = ap.parse_args(argv)
+
+    P, _, _, _ = configs.learned_128(0, 1)
+    net = WarmStartNet(P.columns, P.rows, P.slices, seed=12345)
+    report = {"workload": f"learned-128: random ellipsoid phantoms, {P.columns}x{P.rows}x{P.slices} @ 2 mm, "
+                          f"{args.histories} histories each, MSC hull, FBP x0",
+              "train": args.train, "test": args.test, "target_iterations": args.target_iterations,
+              "ref_iterations": args.ref_iterations}
+    if args.eval_only:
+        net.load(args.weights)
+    else:
+        t0 = time.perf_counter()
+        for s in range(args.train):
+            net.add_sample(*training_pair(s, args.histories, args.target_iterations))
+            if (s + 1) % 25 == 0:
+                print(f"[warmstart] {s + 1}/{args.train} training pairs ({time.perf_counter() - t0:.0f} s)",
+                      flush=True)
+        report["data_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        losses = []
+        chunk = 250
+        for k in range(0, args.steps, chunk):
+            losses.append(net.train(min(chunk, args.steps - k), lr=args.lr, seed=k))
+            print(f"[warmstart] step {k + chunk}: mean loss {losses[-1]:.4e}", flush=True)
+        report["train_s"] = time.perf_counter() - t0
+        report["loss_curve"] = losses
+        if args.weights:
+            net.save(args.weights)
+    evals = []
+    for j in range(args.test):
+        e = evaluate(net, 100000 + j, args.histories, args.ref_iterations)
+        evals.append(e)
+        print(f"[warmstart] test {j}: {json.dumps(e)}", flush=True)
+    sp = [e["speedup"] for e in evals if e["speedup"]]
+    report["evaluations"] = evals
+    report["median_speedup"] = float(np.median(sp)) if sp else None
+    report["median_iterations_net"] = float(np.median([e["iterations_net"] for e in evals])) if evals else None
+    report["net_used"] = sum(e["used_net"] for e in evals)
+    net.close()
+    out = json.dumps(report)
+    if args.report:
+        with open(args.report, "w") as f:
+            f.write(out + "\n")
+    print(out)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

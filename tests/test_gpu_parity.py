@@ -92,6 +92,21 @@ def _run_and_compare(P, data, oracle, case):
         r.reset_iterate()
         r.DROP(P.iterations)
         assert _mismatch(r.image("x").reshape(-1), x) == 0
+        # scheduling (visiting order, XCD slabs, voxels per lane) never changes a bit; the lanes per
+        # history only regroup an FP64 sum of floats
+        for lanes, vox, spatial, slabs in [(16, 1, False, False), (16, 8, True, False), (16, 2, False, True),
+                                           (64, 4, True, True), (4, 4, True, True)]:
+            r.set_launch(lanes, vox, spatial, slabs)
+            r.reset_iterate()
+            r.DROP(P.iterations)
+            xs = r.image("x").reshape(-1)
+            if lanes == 16:
+                assert _mismatch(xs, x) == 0, (lanes, vox, spatial, slabs)
+            else:
+                assert np.linalg.norm(xs - x) <= 1e-7 * np.linalg.norm(x), (lanes, vox, spatial, slabs)
+        r.set_launch()
+        r.reset_iterate()
+        r.DROP(P.iterations)
         res = r.residual()
         ref_res = oracle.residual(ref["offsets"], ref["voxels"], ref["chord"], ref["wepl_recon"], ref["x"],
                                   P.min_mlp_length)
