@@ -112,6 +112,10 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
         int f = std::atoi(e);
         if (f >= 0 && f <= 3) ctx->drop_flags = f;
     }
+    if (const char* e = std::getenv("PCT_DROP_GROUP_ENTRIES")) {
+        long long g = std::atoll(e);
+        if (g >= 1 && g <= (1ll << 30)) ctx->drop_group_entries = g;
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PCT_E_HIP;
@@ -149,8 +153,7 @@ int pct_ctx_destroy(pct_ctx* ctx)
     pct_free(&ctx->T.p26); pct_free(&ctx->T.p312); pct_free(&ctx->T.step_terms);
     pct_free(&ctx->accum64); pct_free(&ctx->accum32); pct_free(&ctx->S); pct_free(&ctx->touched);
     pct_free(&ctx->n_touched); pct_free(&ctx->list_overflow);
-    pct_free(&ctx->plan.t_vox); pct_free(&ctx->plan.t_off); pct_free(&ctx->plan.t_hist); pct_free(&ctx->plan.n_vox);
-    pct_free(&ctx->plan.r); pct_free(&ctx->plan.perm); pct_free(&ctx->plan.desc);
+    pct_drop_plan_free(&ctx->plan);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PCT_OK;

@@ -64,20 +64,33 @@ struct RowDesc {
 // each history through the voxel, in ascending history order).  Arrays of all blocks are
 // concatenated at entry_base (t_off at entry_base + b, one extra element per block).
 
+// Blocks are transposed in groups (one sort per group by (block, voxel)); a group's runs (touched
+// voxels) start at run_base in t_vox / t_off and its entries at rank_base in t_hist.
 struct DropPlan {
     struct Block {
         int64_t start, proc, entry_base, entries;
+        int64_t group, rank_base, run_base;  // run_base: absolute index of the block's first run
+        uint32_t n_vox;                      // touched voxels
+    };
+    struct Group {
+        int64_t first_block, n_blocks, entry_base, entries, hist_begin, hist_end;
     };
     std::vector<Block> blocks;
-    std::vector<uint32_t> n_vox_host;  // per block: number of touched voxels (host copy)
-    uint32_t* t_vox = nullptr;
-    uint32_t* t_off = nullptr;
+    std::vector<Group> groups;
+    uint32_t* t_vox = nullptr;   // per run: voxel
+    uint32_t* t_off = nullptr;   // per run (+1 per group): first entry of the run, relative to the group
     uint32_t* t_hist = nullptr;  // per entry: the visiting rank of the history (ascending history order)
-    uint32_t* n_vox = nullptr;   // per block: number of touched voxels
     uint32_t* perm = nullptr;    // per reconstruction history: in-block index of the history visited at rank k
+    uint32_t* rank = nullptr;    // per reconstruction history: its visiting rank within its block
     RowDesc* desc = nullptr;     // per reconstruction history, in visiting order
     double* r = nullptr;         // per rank of the current block: the update value of that history
-    int64_t cap_entries = 0, cap_off = 0, cap_blocks = 0, cap_r = 0, cap_perm = 0;
+    // scratch of the group transposition (kept between list builds)
+    uint32_t *keys = nullptr, *keys_sorted = nullptr, *vals = nullptr, *runs = nullptr, *n_runs = nullptr;
+    uint32_t *hkey = nullptr, *hkey_sorted = nullptr, *hidx = nullptr, *hidx_sorted = nullptr;
+    uint32_t *lc = nullptr, *pos = nullptr;
+    int32_t* run_start = nullptr;  // per block of a group: index of its first run, -1 if none
+    int64_t cap_entries = 0, cap_off = 0, cap_r = 0, cap_perm = 0, cap_group_e = 0, cap_group_h = 0,
+            cap_group_b = 0;
     int flags = -1;              // PCT_DROP_* flags the plan was built with
     bool ready = false;
 };
@@ -151,8 +164,9 @@ struct pct_ctx {
     // DROP plan (transposed lists) and the scratch of the atomic DROP kernels
     DropPlan plan;
     int drop_row_lanes = 16;   // lanes per history in the row kernel (PCT_DROP_ROW_LANES: 4..64)
-    int drop_col_voxels = 4;   // voxels per lane in the column kernel (PCT_DROP_COL_VOXELS: 1, 2, 4, 8)
+    int drop_col_voxels = 1;   // voxels per lane in the column kernel (PCT_DROP_COL_VOXELS: 1, 2, 4, 8)
     int drop_flags = PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS;  // PCT_DROP_FLAGS
+    int64_t drop_group_entries = 1ll << 30;  // list entries per transposition group (PCT_DROP_GROUP_ENTRIES)
     double* accum64 = nullptr;
     float* accum32 = nullptr;
     uint32_t* S = nullptr;
@@ -212,6 +226,7 @@ struct DropTiming;
 int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing = nullptr);
 int pct_drop_entries(pct_ctx* ctx, int64_t* histories, int64_t* gathered, int64_t* scattered);
 int pct_drop_prepare(pct_ctx* ctx);
+void pct_drop_plan_free(DropPlan* plan);
 int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing);
 int pct_stage_residual(pct_ctx* ctx, double* norm);
 

@@ -256,11 +256,8 @@ extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, d
     DropTiming t;
     PCT_TRY(pct_stage_drop(ctx, iterations, &t));
     PCT_TRY(pct_drop_entries(ctx, histories, gathered, scattered));
-    const size_t nb = ctx->plan.blocks.size();
-    std::vector<uint32_t> nv(nb ? nb : 1);
-    if (nb) PCT_CHECK_HIP(ctx, hipMemcpy(nv.data(), ctx->plan.n_vox, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
     int64_t u = 0;
-    for (size_t b = 0; b < nb; b++) u += nv[b];
+    for (const auto& bk : ctx->plan.blocks) u += bk.n_vox;
     *touched = u;
     *rows_ms = t.scatter_ms / (t.scatter_launches ? t.scatter_launches : 1);
     *cols_ms = t.apply_ms / (t.apply_launches ? t.apply_launches : 1);

@@ -5,10 +5,10 @@
 //     x[v] += (sum over entries e of the block with voxel v of u_i(e)) / S[v],
 //     u_i = L_i * ((b_i - L_i * sum_j x[v_ij]) / (L_i^2 n_i)) * lambda      (DS.cu:9321-9380, 9522-9536)
 // The reference scatters u_i with float atomicAdd into x_update and counts S with atomicAdd
-// (2 atomics per list entry).  Here the block's lists are transposed once, when the lists are built:
-// for every voxel the block touches, the list of its histories in ascending history order (an LSD
-// radix sort by voxel is stable, so equal voxels keep history order).  Then each iteration runs,
-// per block,
+// (2 atomics per list entry).  Here the blocks' lists are transposed once, when the lists are built:
+// for every voxel a block touches, the list of its histories in ascending history order (an LSD
+// radix sort by (block, voxel) is stable, so equal keys keep history order; blocks are sorted in
+// groups of up to 2^30 entries).  Then each iteration runs, per block,
 //   k_drop_rows   G lanes per history: coalesced reads of its voxel list, gather of x, FP64
 //                 reduction of a_i.x, u_i stored in r[rank];
 //   k_drop_cols   one lane per touched voxel: sum of r over its histories in history order
@@ -43,77 +43,96 @@ __device__ __forceinline__ uint32_t slab_index(uint32_t per)
     return per ? (blockIdx.x % kXcds) * per + blockIdx.x / kXcds : blockIdx.x;
 }
 
-// per history of a block: sort key (its sinogram bin) and in-block index
-__global__ void k_hist_keys(int64_t start, int64_t proc, const int32_t* __restrict__ bin_num,
-                            uint32_t* __restrict__ keys, uint32_t* __restrict__ idx)
+// ---- transposition of a group of blocks (once per list build) ----
+// History i belongs to raw block i / BS (BS = DROP_BLOCK_SIZE) at in-block index i % BS and is
+// processed when that index is below the block's proc (quirk A3 may leave a tail unprocessed).
+
+// sort key of a history: (block in group, sinogram bin) with spatial order, (block in group, 0)
+// without; unprocessed histories take the largest bin so they sort to the end of their block.
+__global__ void k_group_hist_keys(int64_t h0, int64_t n, int64_t BS, int64_t rb0, const int32_t* __restrict__ proc_raw,
+                                  const int32_t* __restrict__ bin_num, int spatial, int bin_bits,
+                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ idx)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= proc) return;
-    keys[i] = (uint32_t)bin_num[start + i];
-    idx[i] = (uint32_t)i;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = h0 + t, rb = i / BS;
+    const uint32_t marker = (1u << bin_bits) - 1u;
+    uint32_t b = marker;
+    if (i - rb * BS < proc_raw[rb]) {
+        b = 0u;
+        if (spatial) {
+            const uint32_t bn = (uint32_t)bin_num[i];
+            b = bn < marker ? bn : marker - 1u;
+        }
+    }
+    keys[t] = ((uint32_t)(rb - rb0) << bin_bits) | b;
+    idx[t] = (uint32_t)t;
 }
 
-__global__ void k_iota32(int64_t n, uint32_t* __restrict__ a, uint32_t* __restrict__ b)
+// sorted position s -> visiting rank within the block; perm, rank and the row descriptor
+__global__ void k_group_rank(int64_t h0, int64_t n, int64_t BS, const uint32_t* __restrict__ sorted_idx,
+                             const int64_t* __restrict__ off, const uint32_t* __restrict__ cnt,
+                             const double* __restrict__ chord, const float* __restrict__ wepl,
+                             uint32_t* __restrict__ perm, uint32_t* __restrict__ rank, RowDesc* __restrict__ desc)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    a[i] = (uint32_t)i;
-    b[i] = (uint32_t)i;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = h0 + sorted_idx[s];
+    const int64_t start = (i / BS) * BS;
+    const int64_t k = h0 + s - start;  // the block's histories occupy sorted positions [start - h0, ...)
+    perm[start + k] = (uint32_t)(i - start);
+    rank[i] = (uint32_t)k;
+    RowDesc d;
+    d.off = off[i];
+    d.chord = chord[i];
+    d.cnt = cnt[i];
+    d.wepl = wepl[i];
+    desc[start + k] = d;
 }
 
-// rank[perm[k]] = k
-__global__ void k_rank(int64_t proc, const uint32_t* __restrict__ perm, uint32_t* __restrict__ rank)
+// entries a history contributes to the column sums (0 if unprocessed or shorter than MIN_MLP_LENGTH)
+__global__ void k_group_long_counts(int64_t h0, int64_t n, int64_t BS, const int32_t* __restrict__ proc_raw,
+                                    const uint32_t* __restrict__ cnt, int min_len, uint32_t* __restrict__ out)
 {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= proc) return;
-    rank[perm[k]] = (uint32_t)k;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = h0 + t, rb = i / BS;
+    const uint32_t c = cnt[i];
+    out[t] = (i - rb * BS < proc_raw[rb] && (int)c >= min_len) ? c : 0u;
 }
 
-// per history of a block: number of entries it contributes (0 for paths shorter than MIN_MLP_LENGTH)
-__global__ void k_long_counts(int64_t start, int64_t proc, const uint32_t* __restrict__ cnt, int min_len,
-                              uint32_t* __restrict__ out)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= proc) return;
-    uint32_t c = cnt[start + i];
-    out[i] = (int)c >= min_len ? c : 0u;
-}
-
-// (voxel, rank) pairs in history order; one wavefront per history, coalesced.
-__global__ void k_emit_pairs(int64_t start, int64_t proc, const int64_t* __restrict__ off,
-                             const uint32_t* __restrict__ vox, const uint32_t* __restrict__ long_cnt,
-                             const uint32_t* __restrict__ pos, const uint32_t* __restrict__ rank,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+// ((block in group, voxel), rank) pairs in history order; one wavefront per history, coalesced
+__global__ void k_group_emit(int64_t h0, int64_t n, int64_t BS, int64_t rb0, int vbits,
+                             const int64_t* __restrict__ off, const uint32_t* __restrict__ vox,
+                             const uint32_t* __restrict__ long_cnt, const uint32_t* __restrict__ pos,
+                             const uint32_t* __restrict__ rank, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < proc; i += n_waves) {
-        const uint32_t c = long_cnt[i];
-        const int64_t base = off[start + i];
-        const uint32_t p = pos[i];
-        const uint32_t rk = rank[i];
+    for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n; t += n_waves) {
+        const uint32_t c = long_cnt[t];
+        if (c == 0) continue;
+        const int64_t i = h0 + t;
+        const uint32_t tag = (uint32_t)(i / BS - rb0) << vbits;
+        const int64_t base = off[i];
+        const uint32_t p = pos[t], rk = rank[i];
         for (uint32_t j = lane; j < c; j += kWave) {
-            keys[p + j] = vox[base + j];
+            keys[p + j] = tag | vox[base + j];
             vals[p + j] = rk;
         }
     }
 }
 
-// per rank k of a block: the row descriptor of the history visited at k
-__global__ void k_row_desc(int64_t start, int64_t proc, const uint32_t* __restrict__ perm,
-                           const int64_t* __restrict__ off, const uint32_t* __restrict__ cnt,
-                           const double* __restrict__ chord, const float* __restrict__ wepl,
-                           RowDesc* __restrict__ desc)
+// runs of the sorted keys: voxel of each run, and the first run of every block of the group
+__global__ void k_group_runs(const uint32_t* __restrict__ run_keys, const uint32_t* __restrict__ n_runs, int vbits,
+                             uint32_t* __restrict__ t_vox, int32_t* __restrict__ run_start)
 {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= proc) return;
-    const int64_t p = start + perm[start + k];
-    RowDesc d;
-    d.off = off[p];
-    d.chord = chord[p];
-    d.cnt = cnt[p];
-    d.wepl = wepl[p];
-    desc[start + k] = d;
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= *n_runs) return;
+    const uint32_t key = run_keys[u];
+    const uint32_t lb = key >> vbits;
+    t_vox[u] = key & ((1u << vbits) - 1u);
+    if (u == 0 || (run_keys[u - 1] >> vbits) != lb) run_start[lb] = (int32_t)u;
 }
 
 // per history: u_i (DS.cu:9321-9380 with the list of find_MLP_path_GPU_tabulated).
@@ -256,153 +275,260 @@ inline unsigned slab_grid(int64_t n_logical, bool xcd, uint32_t* per)
     return (unsigned)(p * kXcds);
 }
 
+
 }  // namespace
 
-// Build the per-block visiting order and transposed lists (once per list build).
+namespace {
+
+int bits_for(int64_t n)  // smallest b with 2^b >= n
+{
+    int b = 0;
+    while ((1ll << b) < n) b++;
+    return b;
+}
+
+template <class T>
+int grow(pct_ctx* ctx, T** p, int64_t* cap, int64_t need)
+{
+    if (need <= *cap && *p) return PCT_OK;
+    PCT_TRY(pct_alloc(ctx, p, need));
+    *cap = need;
+    return PCT_OK;
+}
+
+}  // namespace
+
+void pct_drop_plan_free(DropPlan* D)
+{
+    pct_free(&D->t_vox);
+    pct_free(&D->t_off);
+    pct_free(&D->t_hist);
+    pct_free(&D->perm);
+    pct_free(&D->rank);
+    pct_free(&D->desc);
+    pct_free(&D->r);
+    pct_free(&D->keys);
+    pct_free(&D->keys_sorted);
+    pct_free(&D->vals);
+    pct_free(&D->runs);
+    pct_free(&D->n_runs);
+    pct_free(&D->hkey);
+    pct_free(&D->hkey_sorted);
+    pct_free(&D->hidx);
+    pct_free(&D->hidx_sorted);
+    pct_free(&D->lc);
+    pct_free(&D->pos);
+    pct_free(&D->run_start);
+    D->cap_entries = D->cap_off = D->cap_r = D->cap_perm = D->cap_group_e = D->cap_group_h = D->cap_group_b = 0;
+    D->ready = false;
+}
+
+// Build the visiting order and the transposed lists of every block (once per list build).  Blocks
+// are processed in groups of up to 2^30 list entries: per group one radix sort of the histories by
+// (block, sinogram bin) gives the visiting order, and one radix sort of the (block, voxel) keys of
+// all its list entries, a run-length encoding and a scan give every block's touched voxels with
+// their histories in history order (LSD radix sorts are stable).
 int pct_drop_prepare(pct_ctx* ctx)
 {
     const pct_params& P = ctx->P;
     hipStream_t s = ctx->stream;
     const int64_t n = ctx->n_recon;
+    const int64_t BS = P.drop_block_size;
     DropPlan& D = ctx->plan;
     D.blocks.clear();
+    D.groups.clear();
     D.ready = false;
     const bool spatial = (ctx->drop_flags & PCT_DROP_SPATIAL_ORDER) != 0;
+    const int vbits = bits_for((int64_t)P.num_voxels);
+    const int bin_bits = bits_for((int64_t)P.num_bins + 1);
+    if (vbits > 31 || bin_bits > 31) {
+        ctx->err = "DROP transposition: voxel or bin index wider than 31 bits";
+        return PCT_E_LIMIT;
+    }
+    const int block_bits = 32 - (vbits > bin_bits ? vbits : bin_bits);
+    const int64_t max_group_blocks = 1ll << block_bits;
+    const int64_t kGroupEntries = ctx->drop_group_entries, kGroupHistories = 1ll << 30;
+
     // block ranges and entry counts (host pass over the list counts)
     std::vector<uint32_t> cnt(n > 0 ? n : 1);
     if (n > 0) PCT_CHECK_HIP(ctx, hipMemcpy(cnt.data(), ctx->list_cnt, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    int64_t total = 0, max_e = 0, max_proc = 0;
-    for (int64_t start = 0; start < n; start += P.drop_block_size) {
-        int64_t m = n - start < P.drop_block_size ? n - start : P.drop_block_size;
-        int64_t proc = P.quirk_a3_drop_floor ? (m / P.histories_per_block) * P.histories_per_block : m;
+    const int64_t n_raw = (n + BS - 1) / BS;
+    std::vector<int32_t> proc_raw(n_raw > 0 ? n_raw : 1, 0);
+    int64_t total = 0, max_proc = 0;
+    for (int64_t rb = 0; rb < n_raw; rb++) {
+        const int64_t start = rb * BS;
+        const int64_t m = n - start < BS ? n - start : BS;
+        const int64_t proc = P.quirk_a3_drop_floor ? (m / P.histories_per_block) * P.histories_per_block : m;
         if (proc <= 0) continue;
+        proc_raw[rb] = (int32_t)proc;
         int64_t e = 0;
         for (int64_t i = start; i < start + proc; i++)
             if ((int)cnt[i] >= P.min_mlp_length) e += cnt[i];
-        DropPlan::Block bk{start, proc, total, e};
+        if (e >= (1ll << 30)) {  // a group holds at least one whole block
+            ctx->err = "DROP block with 2^30 or more list entries";
+            return PCT_E_LIMIT;
+        }
+        DropPlan::Block bk{};
+        bk.start = start;
+        bk.proc = proc;
+        bk.entry_base = total;
+        bk.entries = e;
+        // group: start a new one when this block would overflow the current one
+        bool fresh = D.groups.empty();
+        if (!fresh) {
+            const DropPlan::Group& g = D.groups.back();
+            const int64_t span = rb - g.hist_begin / BS + 1;  // raw blocks the group would span
+            fresh = g.entries + e > kGroupEntries || span > max_group_blocks || start + m - g.hist_begin > kGroupHistories;
+        }
+        if (fresh) D.groups.push_back(DropPlan::Group{(int64_t)D.blocks.size(), 0, total, 0, start, start});
+        DropPlan::Group& g = D.groups.back();
+        bk.group = (int64_t)D.groups.size() - 1;
+        g.n_blocks = rb - g.hist_begin / BS + 1;
+        g.entries += e;
+        g.hist_end = start + m;
         D.blocks.push_back(bk);
         total += e;
-        max_e = e > max_e ? e : max_e;
         max_proc = proc > max_proc ? proc : max_proc;
     }
-    if (max_e > 0xffffffffll || max_proc > 0xffffffffll) {
-        ctx->err = "DROP block with more than 2^32 list entries";
-        return PCT_E_LIMIT;
+    const int64_t nb = (int64_t)D.blocks.size(), ng = (int64_t)D.groups.size();
+    int64_t max_ge = 0, max_gh = 0, max_gb = 0;
+    for (const auto& g : D.groups) {
+        max_ge = g.entries > max_ge ? g.entries : max_ge;
+        max_gh = g.hist_end - g.hist_begin > max_gh ? g.hist_end - g.hist_begin : max_gh;
+        max_gb = g.n_blocks > max_gb ? g.n_blocks : max_gb;
     }
-    const int64_t nb = (int64_t)D.blocks.size();
-    if (total > D.cap_entries || !D.t_hist) {
-        PCT_TRY(pct_alloc(ctx, &D.t_hist, total));
-        PCT_TRY(pct_alloc(ctx, &D.t_vox, total));
-        D.cap_entries = total;
+    // persistent arrays and scratch
+    PCT_TRY(grow(ctx, &D.t_hist, &D.cap_entries, total + 1));
+    if (D.cap_off < total + ng + 1 || !D.t_vox || !D.t_off) {
+        PCT_TRY(pct_alloc(ctx, &D.t_vox, total + ng + 1));
+        PCT_TRY(pct_alloc(ctx, &D.t_off, total + ng + 1));
+        D.cap_off = total + ng + 1;
     }
-    if (total + nb > D.cap_off || !D.t_off) {
-        PCT_TRY(pct_alloc(ctx, &D.t_off, total + nb));
-        D.cap_off = total + nb;
+    PCT_TRY(grow(ctx, &D.r, &D.cap_r, max_proc + 1));
+    if (D.cap_perm < n + 1 || !D.perm || !D.rank || !D.desc) {
+        PCT_TRY(pct_alloc(ctx, &D.perm, n + 1));
+        PCT_TRY(pct_alloc(ctx, &D.rank, n + 1));
+        PCT_TRY(pct_alloc(ctx, &D.desc, n + 1));
+        D.cap_perm = n + 1;
     }
-    if (nb > D.cap_blocks || !D.n_vox) {
-        PCT_TRY(pct_alloc(ctx, &D.n_vox, nb));
-        D.cap_blocks = nb;
+    if (D.cap_group_e < max_ge + 1 || !D.keys) {
+        PCT_TRY(pct_alloc(ctx, &D.keys, max_ge + 1));
+        PCT_TRY(pct_alloc(ctx, &D.keys_sorted, max_ge + 1));
+        PCT_TRY(pct_alloc(ctx, &D.vals, max_ge + 1));
+        PCT_TRY(pct_alloc(ctx, &D.runs, max_ge + 1));
+        D.cap_group_e = max_ge + 1;
     }
-    if (max_proc > D.cap_r || !D.r) {
-        PCT_TRY(pct_alloc(ctx, &D.r, max_proc));
-        D.cap_r = max_proc;
+    if (D.cap_group_h < max_gh + 1 || !D.hkey) {
+        PCT_TRY(pct_alloc(ctx, &D.hkey, max_gh + 1));
+        PCT_TRY(pct_alloc(ctx, &D.hkey_sorted, max_gh + 1));
+        PCT_TRY(pct_alloc(ctx, &D.hidx, max_gh + 1));
+        PCT_TRY(pct_alloc(ctx, &D.hidx_sorted, max_gh + 1));
+        PCT_TRY(pct_alloc(ctx, &D.lc, max_gh + 1));
+        PCT_TRY(pct_alloc(ctx, &D.pos, max_gh + 1));
+        D.cap_group_h = max_gh + 1;
     }
-    if (n > D.cap_perm || !D.perm || !D.desc) {
-        PCT_TRY(pct_alloc(ctx, &D.perm, n));
-        PCT_TRY(pct_alloc(ctx, &D.desc, n));
-        D.cap_perm = n;
+    if (D.cap_group_b < max_gb + 1 || !D.run_start || !D.n_runs) {
+        PCT_TRY(pct_alloc(ctx, &D.run_start, max_gb + 1));
+        PCT_TRY(pct_alloc(ctx, &D.n_runs, 2));
+        D.cap_group_b = max_gb + 1;
     }
-    // scratch: pair keys/values (max_e + 1); per-history counts, positions, sort keys, ranks (max_proc + 1)
-    uint32_t *keys = nullptr, *keys_sorted = nullptr, *vals = nullptr, *runs = nullptr, *lc = nullptr, *pos = nullptr;
-    uint32_t *hkey = nullptr, *hkey_sorted = nullptr, *hidx = nullptr, *rank = nullptr;
-    PCT_TRY(pct_alloc(ctx, &keys, max_e + 1));
-    PCT_TRY(pct_alloc(ctx, &keys_sorted, max_e + 1));
-    PCT_TRY(pct_alloc(ctx, &vals, max_e + 1));
-    PCT_TRY(pct_alloc(ctx, &runs, max_e + 1));
-    PCT_TRY(pct_alloc(ctx, &lc, max_proc + 1));
-    PCT_TRY(pct_alloc(ctx, &pos, max_proc + 1));
-    PCT_TRY(pct_alloc(ctx, &hkey, max_proc + 1));
-    PCT_TRY(pct_alloc(ctx, &hkey_sorted, max_proc + 1));
-    PCT_TRY(pct_alloc(ctx, &hidx, max_proc + 1));
-    PCT_TRY(pct_alloc(ctx, &rank, max_proc + 1));
-    int end_bit = 1;
-    while ((1ll << end_bit) < (int64_t)P.num_voxels) end_bit++;
-    int bin_bits = 1;
-    while ((1ll << bin_bits) < (int64_t)P.num_bins) bin_bits++;
-    size_t need = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0;
-    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, t1, lc, pos, (int)(max_proc + 1), s));
-    PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, keys, keys_sorted, vals, vals, (int)(max_e + 1),
-                                                          0, end_bit, s));
-    PCT_CHECK_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(nullptr, t3, keys_sorted, keys, runs, D.n_vox,
-                                                             (int)(max_e + 1), s));
-    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, t4, runs, D.t_off, (int)(max_e + 1), s));
-    PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t5, hkey, hkey_sorted, hidx, hidx,
-                                                          (int)(max_proc + 1), 0, 32, s));
-    need = std::max(std::max(t1, t2), std::max(std::max(t3, t4), t5));
-    if (need > ctx->cub_tmp_bytes) {
-        PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
-        ctx->cub_tmp_bytes = need;
+    int32_t* proc_dev = nullptr;
+    PCT_TRY(pct_alloc(ctx, &proc_dev, n_raw + 1));
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(proc_dev, proc_raw.data(), n_raw * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    // library scratch size for the largest group
+    {
+        size_t need = 0, t = 0;
+        const int E1 = (int)(max_ge + 1), H1 = (int)(max_gh + 1);
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t, D.hkey, D.hkey_sorted, D.hidx, D.hidx_sorted,
+                                                              H1, 0, 32, s));
+        need = std::max(need, t);
+        PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, t, D.lc, D.pos, H1, s));
+        need = std::max(need, t);
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t, D.keys, D.keys_sorted, D.vals, D.t_hist, E1, 0,
+                                                              32, s));
+        need = std::max(need, t);
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(nullptr, t, D.keys_sorted, D.keys, D.runs, D.n_runs, E1,
+                                                                 s));
+        need = std::max(need, t);
+        PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, t, D.runs, D.t_off, E1, s));
+        need = std::max(need, t);
+        if (need > ctx->cub_tmp_bytes) {
+            PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
+            ctx->cub_tmp_bytes = need;
+        }
     }
-    for (int64_t bi = 0; bi < nb; bi++) {
-        const DropPlan::Block& bk = D.blocks[bi];
-        const int64_t E = bk.entries, proc = bk.proc;
-        uint32_t* t_hist = D.t_hist + bk.entry_base;
-        uint32_t* t_vox = D.t_vox + bk.entry_base;
-        uint32_t* t_off = D.t_off + bk.entry_base + bi;
-        uint32_t* perm = D.perm + bk.start;
+    std::vector<std::vector<int32_t>> run_start_h(ng);
+    std::vector<uint32_t> n_runs_h(ng, 0);
+    int64_t run_off = 0;
+    std::vector<int64_t> group_run_off(ng, 0);
+    for (int64_t gi = 0; gi < ng; gi++) {
+        const DropPlan::Group& g = D.groups[gi];
+        const int64_t h0 = g.hist_begin, H = g.hist_end - g.hist_begin, E = g.entries;
+        const int64_t rb0 = h0 / BS;
+        const int sort_bits = bits_for(g.n_blocks) + bin_bits;
         size_t tb = ctx->cub_tmp_bytes;
-        // visiting order: perm[k] = in-block index of the history of rank k; rank = inverse
-        if (spatial) {
-            hipLaunchKernelGGL(k_hist_keys, dim3(grid_for(proc)), dim3(kBlock), 0, s, bk.start, proc, ctx->bin_num,
-                               hkey, hidx);
-            PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, tb, hkey, hkey_sorted, hidx, perm,
-                                                                  (int)proc, 0, bin_bits, s));
-            hipLaunchKernelGGL(k_rank, dim3(grid_for(proc)), dim3(kBlock), 0, s, proc, perm, rank);
-        } else {
-            hipLaunchKernelGGL(k_iota32, dim3(grid_for(proc)), dim3(kBlock), 0, s, proc, perm, rank);
+        hipLaunchKernelGGL(k_group_hist_keys, dim3(grid_for(H)), dim3(kBlock), 0, s, h0, H, BS, rb0, proc_dev,
+                           ctx->bin_num, spatial ? 1 : 0, bin_bits, D.hkey, D.hidx);
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, tb, D.hkey, D.hkey_sorted, D.hidx,
+                                                              D.hidx_sorted, (int)H, 0, sort_bits, s));
+        hipLaunchKernelGGL(k_group_rank, dim3(grid_for(H)), dim3(kBlock), 0, s, h0, H, BS, D.hidx_sorted, ctx->list_off,
+                           ctx->list_cnt, ctx->chord, ctx->wepl, D.perm, D.rank, D.desc);
+        group_run_off[gi] = run_off;
+        PCT_CHECK_HIP(ctx, hipMemsetAsync(D.run_start, 0xff, (g.n_blocks + 1) * sizeof(int32_t), s));
+        if (E > 0) {
+            hipLaunchKernelGGL(k_group_long_counts, dim3(grid_for(H)), dim3(kBlock), 0, s, h0, H, BS, proc_dev,
+                               ctx->list_cnt, P.min_mlp_length, D.lc);
+            tb = ctx->cub_tmp_bytes;
+            PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.lc, D.pos, (int)H, s));
+            int64_t wg = (H * kWave + kBlock - 1) / kBlock;
+            if (wg > 65536) wg = 65536;
+            hipLaunchKernelGGL(k_group_emit, dim3((unsigned)wg), dim3(kBlock), 0, s, h0, H, BS, rb0, vbits, ctx->list_off,
+                               ctx->list_vox, D.lc, D.pos, D.rank, D.keys, D.vals);
+            tb = ctx->cub_tmp_bytes;
+            PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, tb, D.keys, D.keys_sorted, D.vals,
+                                                                  D.t_hist + g.entry_base, (int)E, 0,
+                                                                  bits_for(g.n_blocks) + vbits, s));
+            PCT_CHECK_HIP(ctx, hipMemsetAsync(D.runs, 0, (E + 1) * sizeof(uint32_t), s));
+            tb = ctx->cub_tmp_bytes;
+            PCT_CHECK_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(ctx->cub_tmp, tb, D.keys_sorted, D.keys, D.runs,
+                                                                     D.n_runs, (int)E, s));
+            tb = ctx->cub_tmp_bytes;
+            PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.runs, D.t_off + run_off + gi,
+                                                                (int)(E + 1), s));
+            hipLaunchKernelGGL(k_group_runs, dim3(grid_for(E)), dim3(kBlock), 0, s, D.keys, D.n_runs, vbits,
+                               D.t_vox + run_off + gi, D.run_start);
+            PCT_CHECK_HIP(ctx, hipGetLastError());
+            PCT_CHECK_HIP(ctx, hipMemcpyAsync(&n_runs_h[gi], D.n_runs, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
-        hipLaunchKernelGGL(k_row_desc, dim3(grid_for(proc)), dim3(kBlock), 0, s, bk.start, proc, D.perm,
-                           ctx->list_off, ctx->list_cnt, ctx->chord, ctx->wepl, D.desc);
-        if (E == 0) {
-            PCT_CHECK_HIP(ctx, hipMemsetAsync(D.n_vox + bi, 0, sizeof(uint32_t), s));
-            PCT_CHECK_HIP(ctx, hipMemsetAsync(t_off, 0, sizeof(uint32_t), s));
-            continue;
-        }
-        hipLaunchKernelGGL(k_long_counts, dim3(grid_for(proc)), dim3(kBlock), 0, s, bk.start, proc, ctx->list_cnt,
-                           P.min_mlp_length, lc);
-        tb = ctx->cub_tmp_bytes;
-        PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, lc, pos, (int)proc, s));
-        int64_t blocks = (proc * kWave + kBlock - 1) / kBlock;
-        if (blocks > 16384) blocks = 16384;
-        hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)blocks), dim3(kBlock), 0, s, bk.start, proc, ctx->list_off,
-                           ctx->list_vox, lc, pos, rank, keys, vals);
-        tb = ctx->cub_tmp_bytes;
-        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, tb, keys, keys_sorted, vals, t_hist, (int)E,
-                                                              0, end_bit, s));
-        PCT_CHECK_HIP(ctx, hipMemsetAsync(runs, 0, (E + 1) * sizeof(uint32_t), s));
-        tb = ctx->cub_tmp_bytes;
-        PCT_CHECK_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(ctx->cub_tmp, tb, keys_sorted, t_vox, runs,
-                                                                 D.n_vox + bi, (int)E, s));
-        tb = ctx->cub_tmp_bytes;
-        PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, runs, t_off, (int)(E + 1), s));
-        PCT_CHECK_HIP(ctx, hipGetLastError());
+        run_start_h[gi].assign(g.n_blocks + 1, -1);
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(run_start_h[gi].data(), D.run_start, g.n_blocks * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, s));
+        PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));  // run_start and n_runs are reused by the next group
+        run_off += (int64_t)n_runs_h[gi];
     }
-    D.n_vox_host.assign(nb > 0 ? nb : 1, 0);
-    if (nb > 0)
-        PCT_CHECK_HIP(ctx, hipMemcpyAsync(D.n_vox_host.data(), D.n_vox, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    pct_free(&keys);
-    pct_free(&keys_sorted);
-    pct_free(&vals);
-    pct_free(&runs);
-    pct_free(&lc);
-    pct_free(&pos);
-    pct_free(&hkey);
-    pct_free(&hkey_sorted);
-    pct_free(&hidx);
-    pct_free(&rank);
+    pct_free(&proc_dev);
+    // per block: its runs [run_start, next run_start) within the group
+    for (auto& bk : D.blocks) {
+        const DropPlan::Group& g = D.groups[bk.group];
+        const int64_t lb = bk.start / BS - g.hist_begin / BS;
+        const std::vector<int32_t>& rs = run_start_h[bk.group];
+        bk.rank_base = g.entry_base;
+        bk.n_vox = 0;
+        bk.run_base = group_run_off[bk.group] + bk.group;
+        if (rs[lb] < 0) continue;
+        int64_t end = n_runs_h[bk.group];
+        for (int64_t j = lb + 1; j < g.n_blocks; j++)
+            if (rs[j] >= 0) {
+                end = rs[j];
+                break;
+            }
+        bk.run_base += rs[lb];
+        bk.n_vox = (uint32_t)(end - rs[lb]);
+    }
     D.flags = ctx->drop_flags;
     D.ready = true;
+    (void)nb;
     return PCT_OK;
 }
 
@@ -440,14 +566,14 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
         }
 #undef PCT_ROWS
         if (timing) mark();
-        const uint32_t U = D.n_vox_host[bi];
+        const uint32_t U = bk.n_vox;
         if (bk.entries > 0 && U > 0) {
             uint32_t cper = 0;
             const int64_t vpw = (int64_t)kBlock * V;  // voxels per workgroup
             const unsigned cgrid = slab_grid((U + vpw - 1) / vpw, xcd, &cper);
-            const uint32_t* t_vox = D.t_vox + bk.entry_base;
-            const uint32_t* t_off = D.t_off + bk.entry_base + bi;
-            const uint32_t* t_rank = D.t_hist + bk.entry_base;
+            const uint32_t* t_vox = D.t_vox + bk.run_base;
+            const uint32_t* t_off = D.t_off + bk.run_base;
+            const uint32_t* t_rank = D.t_hist + bk.rank_base;
             dim3 cg(cgrid), cbk(kBlock);
 #define PCT_COLS(A, VV) \
     hipLaunchKernelGGL((k_drop_cols<A, VV>), cg, cbk, 0, s, U, cper, t_vox, t_off, t_rank, D.r, ctx->x)

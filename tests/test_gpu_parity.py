@@ -123,6 +123,27 @@ def test_parity_head_msc(oracle):
     _run_and_compare(P, data, oracle, "head-MSC")
 
 
+def test_drop_transposition_groups_do_not_change_bits(monkeypatch):
+    """Blocks are transposed in groups of up to 2^30 entries; forcing many small groups (one to a few
+    blocks each, including the quirk-A3 block tails) must give the same image bit for bit."""
+    from pct_reconstruction_amd import Reconstruction
+    P, data = small_head(n=60000, seed=5, drop_block_size=5000, histories_per_block=640, iterations=3)
+    images = []
+    for group_entries in (None, "150000", "1"):
+        if group_entries is None:
+            monkeypatch.delenv("PCT_DROP_GROUP_ENTRIES", raising=False)
+        else:
+            monkeypatch.setenv("PCT_DROP_GROUP_ENTRIES", group_entries)
+        with Reconstruction(P) as r:
+            r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+            r.run()
+            images.append(r.image("x").reshape(-1))
+            x0 = r.image("x0").reshape(-1)
+    assert _mismatch(images[1], images[0]) == 0
+    assert _mismatch(images[2], images[0]) == 0
+    assert np.linalg.norm(images[0] - x0) > 0  # DROP changed the image
+
+
 @pytest.mark.parametrize("accum", [0, 2])
 def test_parity_drop_accumulation_modes(oracle, accum):
     P, data = small_cylinder(n=40000, seed=21, drop_accum=accum, iterations=3)
