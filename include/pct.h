@@ -39,11 +39,13 @@ enum pct_x0_type { PCT_X0_IMPORT = 0, PCT_X0_HULL = 1, PCT_X0_FBP = 2, PCT_X0_HY
 /* DROP per-voxel accumulation precision (the reference uses float atomicAdd, DS.cu:9533):
  * FP32 sums in history order (reference precision), FP64 sums, or FP32 with Kahan compensation. */
 enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1, PCT_ACCUM_KAHAN32 = 2 };
-/* DROP scheduling flags (pct_set_launch).  Neither changes any result bit:
+/* DROP scheduling flags (pct_set_launch).  None changes any result bit:
  * SPATIAL_ORDER visits a block's histories sorted by sinogram bin (v, angle, t) so that a wavefront
  * gathers along neighbouring parallel paths; XCD_SLABS gives each of the 8 XCDs a contiguous slab
- * of those histories (row kernel) and of the touched voxels (column kernel). */
-enum pct_drop_flags { PCT_DROP_SPATIAL_ORDER = 1, PCT_DROP_XCD_SLABS = 2 };
+ * of those histories (row kernel) and of the touched voxels (column kernel); BRICKS keeps the image
+ * in 4x4x2-voxel bricks (one 128-byte line each) during DROP, so consecutive voxels of a path and
+ * neighbouring paths share cache lines. */
+enum pct_drop_flags { PCT_DROP_SPATIAL_ORDER = 1, PCT_DROP_XCD_SLABS = 2, PCT_DROP_BRICKS = 4 };
 
 /*
  * Parameters.  Field names follow the settings.cfg keys (settings.cfg:1-164) and the
@@ -93,6 +95,14 @@ typedef struct pct_params {
     int32_t quirk_a3_drop_floor;  /* DROP block processes floor(n/HISTORIES_PER_BLOCK)*HPB (DS.cu:9558) */
     int32_t quirk_voxel_decode;   /* voxel_2_3D_voxels '>' decode used by FBP (DS.cu:11124-11134) */
     int32_t drop_accum;           /* enum pct_drop_accum */
+    /* optional image filters, off by default (FBP() DS.cu:6080-6168, hull_selection :7375-7386) */
+    int32_t median_filter_fbp;       /* MEDIAN_FILTER_FBP: median of the (2r+1)^2 in-slice window -> x0 */
+    int32_t fbp_med_filter_radius;   /* FBP_MED_FILTER_RADIUS r (0..10) */
+    int32_t avg_filter_fbp;          /* AVG_FILTER_FBP: takes precedence over the median filter (else-if) */
+    int32_t avg_filter_hull;         /* AVG_FILTER_HULL: thresholded window average of the hull -> endpoints */
+    int32_t hull_avg_filter_radius;  /* HULL_AVG_FILTER_RADIUS */
+    int32_t reserved0;
+    double hull_avg_filter_threshold; /* HULL_AVG_FILTER_THRESHOLD (DS.h:483, 0.1) */
 
     /* ---- derived (pct_params_derive) ---- */
     int32_t t_bins, v_bins, angular_bins, num_bins;
@@ -185,7 +195,9 @@ int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, double* cols
 /* ---------------------------------------------------------------- outputs ---- */
 enum pct_image_kind { PCT_IMG_X = 0, PCT_IMG_FBP = 1, PCT_IMG_X0 = 2, PCT_IMG_HULL = 3,
                       PCT_IMG_SC_HULL = 4, PCT_IMG_MSC_COUNTS = 5, PCT_IMG_SM_COUNTS = 6,
-                      PCT_IMG_MSC_HULL = 7, PCT_IMG_SM_HULL = 8, PCT_IMG_FBP_HULL = 9 };
+                      PCT_IMG_MSC_HULL = 7, PCT_IMG_SM_HULL = 8, PCT_IMG_FBP_HULL = 9,
+                      PCT_IMG_FBP_MEDIAN = 10,    /* median-filtered FBP (MEDIAN_FILTER_FBP) */
+                      PCT_IMG_HULL_FILTERED = 11  /* average-filtered hull (AVG_FILTER_HULL) */ };
 /* Copy an image (NUM_VOXELS values, index x + y*COLUMNS + z*COLUMNS*ROWS) to host as float. */
 int pct_get_image(pct_ctx* ctx, int kind, float* out);
 /* Overwrite the current iterate x (NUM_VOXELS floats), e.g. a learned warm start. */

@@ -600,6 +600,48 @@ void o_backproject(const pct_params* p, const float* sf, float* image)
     }
 }
 
+/* median_filter_2D (DS.cu:7482-7519) as FBP() applies it (:6150-6166): interior voxels get the
+ * element at position (2r+1)^2/2 of their sorted in-slice window, the rest 0. */
+static int o_cmp_float(const void* a, const void* b)
+{
+    float x = *(const float*)a, y = *(const float*)b;
+    return (x > y) - (x < y);
+}
+void o_median_filter_2d(const pct_params* p, const float* in, int radius, float* out)
+{
+    const int C = p->columns, R = p->rows, S = p->slices;
+    const int w = 2 * radius + 1, n = w * w;
+    float* nb = (float*)malloc(sizeof(float) * n);
+    memset(out, 0, sizeof(float) * (size_t)C * R * S);
+    for (int s = 0; s < S; s++)
+        for (int c = radius; c < C - radius; c++)
+            for (int r = radius; r < R - radius; r++) {
+                int k = 0;
+                for (int cc = c - radius; cc <= c + radius; cc++)
+                    for (int rr = r - radius; rr <= r + radius; rr++) nb[k++] = in[cc + rr * C + (int64_t)s * C * R];
+                qsort(nb, n, sizeof(float), o_cmp_float);
+                out[c + r * C + (int64_t)s * C * R] = nb[n / 2];
+            }
+    free(nb);
+}
+
+/* averaging_filter_GPU with perform_threshold (DS.cu:7420-7443), on a 0/1 hull */
+void o_avg_filter_hull(const pct_params* p, const uint8_t* in, int radius, double threshold, uint8_t* out)
+{
+    const int C = p->columns, R = p->rows, S = p->slices;
+    for (int s = 0; s < S; s++)
+        for (int y = 0; y < R; y++)
+            for (int x = 0; x < C; x++) {
+                int left = imax(x - radius, 0), right = x + radius < C - 1 ? x + radius : C - 1;
+                int top = imax(y - radius, 0), bottom = y + radius < R - 1 ? y + radius : R - 1;
+                int nvox = (right - left + 1) * (bottom - top + 1);
+                double sum_threshold = nvox * threshold, sum = 0.0;
+                for (int c = left; c <= right; c++)
+                    for (int r = top; r <= bottom; r++) sum += in[c + r * C + (int64_t)s * C * R] ? 1.0 : 0.0;
+                out[x + y * C + (int64_t)s * C * R] = sum > sum_threshold;
+            }
+}
+
 /* FBP_image_2_hull_GPU, DS.cu:6592-6603 */
 void o_fbp_hull(const pct_params* p, const float* image, uint8_t* hull)
 {

@@ -63,6 +63,8 @@ void o_filter(const pct_params* p, const float* sinogram, float* filtered);
 void o_backproject(const pct_params* p, const float* filtered, float* image);
 /* FBP_image_2_hull_GPU (:6592). */
 void o_fbp_hull(const pct_params* p, const float* image, uint8_t* hull);
+void o_median_filter_2d(const pct_params* p, const float* in, int radius, float* out);
+void o_avg_filter_hull(const pct_params* p, const uint8_t* in, int radius, double threshold, uint8_t* out);
 
 /* MLP lookup tables (:8421-8502), FP64. */
 typedef struct o_tables {

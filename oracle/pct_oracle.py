@@ -46,6 +46,8 @@ def lib() -> C.CDLL:
         "o_filter": ([P(Params), vp, vp], None),
         "o_backproject": ([P(Params), vp, vp], None),
         "o_fbp_hull": ([P(Params), vp, vp], None),
+        "o_median_filter_2d": ([P(Params), vp, C.c_int, vp], None),
+        "o_avg_filter_hull": ([P(Params), vp, C.c_int, C.c_double, vp], None),
         "o_tables_create": ([P(Params), vp], C.c_int),
         "o_tables_destroy": ([vp], None),
         "o_endpoints": ([P(Params), vp, i64, f10, vp], None),
@@ -180,6 +182,19 @@ def fbp_hull(P, image):
     return out
 
 
+def median_filter_2d(P, image, radius):
+    out = np.zeros(P.num_voxels, np.float32)
+    lib().o_median_filter_2d(C.byref(P), _p(np.ascontiguousarray(image, np.float32).reshape(-1)), int(radius), _p(out))
+    return out
+
+
+def avg_filter_hull(P, hull, radius, threshold):
+    out = np.zeros(P.num_voxels, np.uint8)
+    lib().o_avg_filter_hull(C.byref(P), _p(np.ascontiguousarray(hull, np.uint8).reshape(-1)), int(radius),
+                            float(threshold), _p(out))
+    return out
+
+
 def endpoints(P, hull_u8, h):
     h = np.ascontiguousarray(h.copy(), np.float32)
     n = h.shape[1]
@@ -291,24 +306,31 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     if P.fbp_on:
         out["sinogram_filtered"] = filter_sinogram(P, sino)
         out["fbp"] = backproject(P, out["sinogram_filtered"])
+        fbp_x0 = out["fbp"]
+        if P.median_filter_fbp and P.fbp_med_filter_radius > 0 and not P.avg_filter_fbp:  # DS.cu:6150
+            fbp_x0 = out["fbp_median"] = median_filter_2d(P, out["fbp"], P.fbp_med_filter_radius)
         if P.hull_type == 4:
             out["fbp_hull"] = fbp_hull(P, out["fbp"])
     hull = {1: lambda: out["sc_hull"], 2: lambda: (out["msc_hull"] == 1).astype(np.uint8),
             3: lambda: (out["sm_hull"] == 1).astype(np.uint8), 4: lambda: out["fbp_hull"]}[P.hull_type]()
     hull = np.ascontiguousarray(hull, np.uint8)
     out["hull"] = hull
+    hull_ep = hull  # the device hull used by the endpoints (DS.cu:7375-7386)
+    if P.avg_filter_hull and P.hull_avg_filter_radius > 0:
+        hull_ep = out["hull_filtered"] = avg_filter_hull(P, hull, P.hull_avg_filter_radius,
+                                                         P.hull_avg_filter_threshold)
     if P.x_0_type == 1:
         x0 = hull.astype(np.float32)
     elif P.x_0_type == 2:
-        x0 = out["fbp"].copy()
+        x0 = fbp_x0.copy()
     elif P.x_0_type == 3:
-        x0 = np.where(hull != 0, out["fbp"], np.float32(0)).astype(np.float32)
+        x0 = np.where(hull != 0, fbp_x0, np.float32(0)).astype(np.float32)
     else:
         x0 = np.zeros(P.num_voxels, np.float32)
     out["x0"] = x0
     if stop_after == "fbp":
         return out
-    h2, first = endpoints(P, hull, h)
+    h2, first = endpoints(P, hull_ep, h)
     keep = first != P.num_voxels
     h2 = np.ascontiguousarray(h2[:, keep])
     w2 = np.ascontiguousarray(w[keep])

@@ -21,10 +21,11 @@ HULL_IMPORT, HULL_SC, HULL_MSC, HULL_SM, HULL_FBP = 0, 1, 2, 3, 4
 FILTER_RAM_LAK, FILTER_SHEPP_LOGAN, FILTER_NONE = 0, 1, 2
 X0_IMPORT, X0_HULL, X0_FBP, X0_HYBRID, X0_ZEROS = 0, 1, 2, 3, 4
 ACCUM_FP32, ACCUM_FP64, ACCUM_KAHAN32 = 0, 1, 2
-DROP_SPATIAL_ORDER, DROP_XCD_SLABS = 1, 2
+DROP_SPATIAL_ORDER, DROP_XCD_SLABS, DROP_BRICKS = 1, 2, 4
+DROP_DEFAULT_FLAGS = DROP_SPATIAL_ORDER | DROP_BRICKS  # the context's default (internal.h)
 
 IMG = {"x": 0, "fbp": 1, "x0": 2, "hull": 3, "sc_hull": 4, "msc_counts": 5, "sm_counts": 6, "msc_hull": 7,
-       "sm_hull": 8, "fbp_hull": 9}
+       "sm_hull": 8, "fbp_hull": 9, "fbp_median": 10, "hull_filtered": 11}
 BINS = {"counts_pre": 0, "mean_wepl": 1, "mean_ut": 2, "mean_uv": 3, "std_wepl": 4, "std_ut": 5, "std_uv": 6,
         "counts_post": 7, "sinogram": 8, "sinogram_filtered": 9}
 HIST = {"x_entry": 0, "y_entry": 1, "z_entry": 2, "x_exit": 3, "y_exit": 4, "z_exit": 5, "xy_entry_angle": 6,
@@ -52,7 +53,9 @@ class Params(C.Structure):
         ("trig_table_step_degrees", _D), ("coeff_table_range", _D), ("coeff_table_step", _D),
         ("poly_table_range", _D), ("poly_table_step", _D), ("e_0", _D), ("x_0_rad_length", _D), ("rsp_air", _D),
         ("source_radius", _D), ("data_in_mm", _I), ("quirk_a1_bin0", _I), ("quirk_a3_drop_floor", _I),
-        ("quirk_voxel_decode", _I), ("drop_accum", _I),
+        ("quirk_voxel_decode", _I), ("drop_accum", _I), ("median_filter_fbp", _I), ("fbp_med_filter_radius", _I),
+        ("avg_filter_fbp", _I), ("avg_filter_hull", _I), ("hull_avg_filter_radius", _I), ("reserved0", _I),
+        ("hull_avg_filter_threshold", _D),
         # derived
         ("t_bins", _I), ("v_bins", _I), ("angular_bins", _I), ("num_bins", _I), ("slices", _I), ("num_voxels", _I),
         ("recon_cyl_height", _D), ("image_width", _D), ("image_height", _D), ("image_thickness", _D),

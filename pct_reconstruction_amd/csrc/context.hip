@@ -110,7 +110,7 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
     }
     if (const char* e = std::getenv("PCT_DROP_FLAGS")) {
         int f = std::atoi(e);
-        if (f >= 0 && f <= 3) ctx->drop_flags = f;
+        if (f >= 0 && f <= 7) ctx->drop_flags = f;
     }
     if (const char* e = std::getenv("PCT_DROP_GROUP_ENTRIES")) {
         long long g = std::atoll(e);
@@ -147,6 +147,7 @@ int pct_ctx_destroy(pct_ctx* ctx)
     pct_free(&ctx->sc_carved); pct_free(&ctx->sc_hull); pct_free(&ctx->msc_counts); pct_free(&ctx->sm_counts);
     pct_free(&ctx->msc_hull); pct_free(&ctx->sm_hull); pct_free(&ctx->fbp_hull); pct_free(&ctx->hull);
     pct_free(&ctx->hull_import); pct_free(&ctx->fbp); pct_free(&ctx->x0); pct_free(&ctx->x);
+    pct_free(&ctx->fbp_x0); pct_free(&ctx->hull_endpoints);
     pct_free(&ctx->list_off); pct_free(&ctx->list_cnt); pct_free(&ctx->list_vox); pct_free(&ctx->chord);
     pct_free(&ctx->cap_scratch);
     pct_free(&ctx->T.sin_t); pct_free(&ctx->T.cos_t); pct_free(&ctx->T.scat); pct_free(&ctx->T.p12);
@@ -302,7 +303,7 @@ int pct_initial_iterate(pct_ctx* ctx, const float* x0_import)
             return PCT_E_STATE;
         }
         hipLaunchKernelGGL(k_x0, dim3(grid_for(NV)), dim3(kBlock), 0, ctx->stream, NV, P.x_0_type, ctx->hull,
-                           ctx->fbp, ctx->x0);
+                           ctx->fbp_x0 ? ctx->fbp_x0 : ctx->fbp, ctx->x0);
     }
     PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->x, ctx->x0, NV * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     PCT_CHECK_HIP(ctx, hipGetLastError());
@@ -375,6 +376,8 @@ int pct_get_image(pct_ctx* ctx, int kind, float* out)
     case PCT_IMG_MSC_HULL: src = ctx->msc_hull; type = 2; break;
     case PCT_IMG_SM_HULL: src = ctx->sm_hull; type = 2; break;
     case PCT_IMG_FBP_HULL: src = ctx->fbp_hull; type = 1; break;
+    case PCT_IMG_FBP_MEDIAN: src = ctx->fbp_x0; break;
+    case PCT_IMG_HULL_FILTERED: src = ctx->hull_endpoints; type = 1; break;
     default: return PCT_E_ARG;
     }
     if (!src) {
@@ -578,7 +581,7 @@ extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_vox
     if (!ctx) return PCT_E_ARG;
     const int g = drop_row_lanes, v = drop_col_voxels;
     if (!(g == 4 || g == 8 || g == 16 || g == 32 || g == 64) || !(v == 1 || v == 2 || v == 4 || v == 8)) return PCT_E_ARG;
-    if (drop_flags & ~(PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS)) return PCT_E_ARG;
+    if (drop_flags & ~(PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS | PCT_DROP_BRICKS)) return PCT_E_ARG;
     ctx->drop_row_lanes = g;
     ctx->drop_col_voxels = v;
     ctx->drop_flags = drop_flags;

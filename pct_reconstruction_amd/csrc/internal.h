@@ -61,11 +61,9 @@ struct RowDesc {
 // Per-block transposed lists for the atomic-free DROP (k_drop2.hip).  Block b covers histories
 // [start, start + proc); its `entries` list entries (histories with >= MIN_MLP_LENGTH voxels) are
 // grouped by voxel: t_vox[u] touched voxel, t_off[u..u+1] its range in t_hist (visiting rank of
-// each history through the voxel, in ascending history order).  Arrays of all blocks are
-// concatenated at entry_base (t_off at entry_base + b, one extra element per block).
-
-// Blocks are transposed in groups (one sort per group by (block, voxel)); a group's runs (touched
-// voxels) start at run_base in t_vox / t_off and its entries at rank_base in t_hist.
+// each history through the voxel, in ascending history order).  Blocks are transposed in groups
+// (one sort per group by (block, voxel)): a block's runs start at run_base in t_vox / t_off, and
+// t_off counts entries from its group's first entry, rank_base in t_hist.
 struct DropPlan {
     struct Block {
         int64_t start, proc, entry_base, entries;
@@ -89,6 +87,11 @@ struct DropPlan {
     uint32_t *hkey = nullptr, *hkey_sorted = nullptr, *hidx = nullptr, *hidx_sorted = nullptr;
     uint32_t *lc = nullptr, *pos = nullptr;
     int32_t* run_start = nullptr;  // per block of a group: index of its first run, -1 if none
+    // brick layout (PCT_DROP_BRICKS): list entries and image in 4x4x2 bricks of 32 voxels
+    uint32_t* vox_b = nullptr;     // list entries as brick indices (same CSR layout as list_vox)
+    float* x_b = nullptr;          // image in brick order during DROP
+    int64_t cap_vox_b = 0, cap_x_b = 0, nv_b = 0;
+    int nbx = 0, nby = 0;
     int64_t cap_entries = 0, cap_off = 0, cap_r = 0, cap_perm = 0, cap_group_e = 0, cap_group_h = 0,
             cap_group_b = 0;
     int flags = -1;              // PCT_DROP_* flags the plan was built with
@@ -149,6 +152,8 @@ struct pct_ctx {
     uint8_t* hull = nullptr;
     uint8_t* hull_import = nullptr;
     float* fbp = nullptr;
+    float* fbp_x0 = nullptr;          // median-filtered FBP (MEDIAN_FILTER_FBP), feeds x0 only
+    uint8_t* hull_endpoints = nullptr; // average-filtered hull (AVG_FILTER_HULL), feeds the MLP endpoints
     float* x0 = nullptr;
     float* x = nullptr;
 
@@ -165,7 +170,7 @@ struct pct_ctx {
     DropPlan plan;
     int drop_row_lanes = 16;   // lanes per history in the row kernel (PCT_DROP_ROW_LANES: 4..64)
     int drop_col_voxels = 1;   // voxels per lane in the column kernel (PCT_DROP_COL_VOXELS: 1, 2, 4, 8)
-    int drop_flags = PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS;  // PCT_DROP_FLAGS
+    int drop_flags = PCT_DROP_SPATIAL_ORDER | PCT_DROP_BRICKS;  // PCT_DROP_FLAGS
     int64_t drop_group_entries = 1ll << 30;  // list entries per transposition group (PCT_DROP_GROUP_ENTRIES)
     double* accum64 = nullptr;
     float* accum32 = nullptr;
@@ -226,7 +231,10 @@ struct DropTiming;
 int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing = nullptr);
 int pct_drop_entries(pct_ctx* ctx, int64_t* histories, int64_t* gathered, int64_t* scattered);
 int pct_drop_prepare(pct_ctx* ctx);
+int pct_median_filter_fbp(pct_ctx* ctx);
+int pct_avg_filter_hull(pct_ctx* ctx);
 void pct_drop_plan_free(DropPlan* plan);
+int pct_drop_enter(pct_ctx* ctx, bool enter);  // image into (true) / out of the DROP layout
 int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing);
 int pct_stage_residual(pct_ctx* ctx, double* norm);
 

@@ -214,7 +214,9 @@ int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* c
 int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing)
 {
     if (!ctx->plan.ready || ctx->plan.flags != ctx->drop_flags) PCT_TRY(pct_drop_prepare(ctx));
+    PCT_TRY(pct_drop_enter(ctx, true));
     for (int it = 0; it < iterations; it++) PCT_TRY(pct_drop_iterate_plan(ctx, timing));
+    PCT_TRY(pct_drop_enter(ctx, false));
     return PCT_OK;
 }
 

@@ -26,6 +26,7 @@
 // u_i is stored; the column sums still run in history order.  Workgroups are dispatched round-robin
 // over the 8 XCDs; remapping the workgroup index gives each XCD one contiguous slab of ranks (rows)
 // or of touched voxels (columns), so each XCD's L2 holds a slab of x instead of all of it.
+#include <hip/hip_ext.h>
 #include <hipcub/hipcub.hpp>
 
 #include "internal.h"
@@ -41,6 +42,38 @@ constexpr unsigned kXcds = 8;  // MI355X: 8 XCDs of 32 CUs; workgroup w runs on 
 __device__ __forceinline__ uint32_t slab_index(uint32_t per)
 {
     return per ? (blockIdx.x % kXcds) * per + blockIdx.x / kXcds : blockIdx.x;
+}
+
+// Brick layout: voxel (x, y, z) of the C x R x S grid -> brick ((z/2, y/4, x/4) in a grid of
+// nbx x nby columns of bricks) * 32 + (z%2, y%4, x%4).  32 floats = one 128-byte line.
+struct BrickGeom {
+    uint32_t C, R, nbx, nby;
+};
+__device__ __forceinline__ uint32_t brick_of(uint32_t v, BrickGeom g)
+{
+    const uint32_t x = v % g.C, t = v / g.C, y = t % g.R, z = t / g.R;
+    const uint32_t b = ((z >> 1) * g.nby + (y >> 2)) * g.nbx + (x >> 2);
+    return b * 32u + (z & 1u) * 16u + (y & 3u) * 4u + (x & 3u);
+}
+
+// list entries (all `n` of the CSR capacity; unused slots hold no voxel and are mapped harmlessly)
+__global__ void k_bricks_list(int64_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, BrickGeom g,
+                              uint32_t nv)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = in[i];
+    out[i] = v < nv ? brick_of(v, g) : 0u;
+}
+
+// image to (to_bricks = 1) or from brick order
+__global__ void k_x_bricks(int64_t nv, float* __restrict__ x, float* __restrict__ xb, BrickGeom g, int to_bricks)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const uint32_t b = brick_of((uint32_t)v, g);
+    if (to_bricks) xb[b] = x[v];
+    else x[v] = xb[b];
 }
 
 // ---- transposition of a group of blocks (once per list build) ----
@@ -319,6 +352,9 @@ void pct_drop_plan_free(DropPlan* D)
     pct_free(&D->lc);
     pct_free(&D->pos);
     pct_free(&D->run_start);
+    pct_free(&D->vox_b);
+    pct_free(&D->x_b);
+    D->cap_vox_b = D->cap_x_b = 0;
     D->cap_entries = D->cap_off = D->cap_r = D->cap_perm = D->cap_group_e = D->cap_group_h = D->cap_group_b = 0;
     D->ready = false;
 }
@@ -339,7 +375,15 @@ int pct_drop_prepare(pct_ctx* ctx)
     D.groups.clear();
     D.ready = false;
     const bool spatial = (ctx->drop_flags & PCT_DROP_SPATIAL_ORDER) != 0;
-    const int vbits = bits_for((int64_t)P.num_voxels);
+    const bool bricks = (ctx->drop_flags & PCT_DROP_BRICKS) != 0;
+    int64_t nv_index = P.num_voxels;  // index space of the voxel keys
+    if (bricks) {
+        D.nbx = (P.columns + 3) / 4;
+        D.nby = (P.rows + 3) / 4;
+        D.nv_b = (int64_t)D.nbx * D.nby * ((P.slices + 1) / 2) * 32;
+        nv_index = D.nv_b;
+    }
+    const int vbits = bits_for(nv_index);
     const int bin_bits = bits_for((int64_t)P.num_bins + 1);
     if (vbits > 31 || bin_bits > 31) {
         ctx->err = "DROP transposition: voxel or bin index wider than 31 bits";
@@ -432,6 +476,23 @@ int pct_drop_prepare(pct_ctx* ctx)
         PCT_TRY(pct_alloc(ctx, &D.n_runs, 2));
         D.cap_group_b = max_gb + 1;
     }
+    const uint32_t* lvox = ctx->list_vox;
+    if (bricks) {
+        if (D.cap_vox_b < ctx->list_cap + 1 || !D.vox_b) {
+            PCT_TRY(pct_alloc(ctx, &D.vox_b, ctx->list_cap + 1));
+            D.cap_vox_b = ctx->list_cap + 1;
+        }
+        if (D.cap_x_b < D.nv_b || !D.x_b) {
+            PCT_TRY(pct_alloc(ctx, &D.x_b, D.nv_b));
+            D.cap_x_b = D.nv_b;
+            PCT_CHECK_HIP(ctx, hipMemsetAsync(D.x_b, 0, D.nv_b * sizeof(float), s));  // padding stays 0
+        }
+        const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
+        if (ctx->list_cap > 0)
+            hipLaunchKernelGGL(k_bricks_list, dim3(grid_for(ctx->list_cap)), dim3(kBlock), 0, s, ctx->list_cap,
+                               ctx->list_vox, D.vox_b, bg, (uint32_t)P.num_voxels);
+        lvox = D.vox_b;
+    }
     int32_t* proc_dev = nullptr;
     PCT_TRY(pct_alloc(ctx, &proc_dev, n_raw + 1));
     PCT_CHECK_HIP(ctx, hipMemcpyAsync(proc_dev, proc_raw.data(), n_raw * sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -483,7 +544,7 @@ int pct_drop_prepare(pct_ctx* ctx)
             int64_t wg = (H * kWave + kBlock - 1) / kBlock;
             if (wg > 65536) wg = 65536;
             hipLaunchKernelGGL(k_group_emit, dim3((unsigned)wg), dim3(kBlock), 0, s, h0, H, BS, rb0, vbits, ctx->list_off,
-                               ctx->list_vox, D.lc, D.pos, D.rank, D.keys, D.vals);
+                               lvox, D.lc, D.pos, D.rank, D.keys, D.vals);
             tb = ctx->cub_tmp_bytes;
             PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, tb, D.keys, D.keys_sorted, D.vals,
                                                                   D.t_hist + g.entry_base, (int)E, 0,
@@ -532,51 +593,71 @@ int pct_drop_prepare(pct_ctx* ctx)
     return PCT_OK;
 }
 
-// One DROP iteration with the prepared plan; optional per-launch timing.
+// Launch `kernel`.  With `ev`, through hipExtLaunchKernelGGL with events recorded at the kernel's own
+// start and end (the duration a kernel trace reports), appended to *ev as a pair.
+template <typename K, typename... Args>
+void drop_launch(K kernel, dim3 g, dim3 b, hipStream_t s, std::vector<hipEvent_t>* ev, Args... args)
+{
+    if (ev) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        hipExtLaunchKernelGGL(kernel, g, b, 0, s, e0, e1, 0, args...);
+        ev->push_back(e0);
+        ev->push_back(e1);
+    } else {
+        hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
+    }
+}
+
+// Image into / out of brick order around a DROP call (no-op without PCT_DROP_BRICKS).
+int pct_drop_enter(pct_ctx* ctx, bool enter)
+{
+    const DropPlan& D = ctx->plan;
+    if (!(D.flags & PCT_DROP_BRICKS)) return PCT_OK;
+    const pct_params& P = ctx->P;
+    const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
+    hipLaunchKernelGGL(k_x_bricks, dim3(grid_for(P.num_voxels)), dim3(kBlock), 0, ctx->stream, (int64_t)P.num_voxels,
+                       ctx->x, D.x_b, bg, enter ? 1 : 0);
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+// One DROP iteration with the prepared plan; optional per-kernel timing.
 int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
 {
     const pct_params& P = ctx->P;
     hipStream_t s = ctx->stream;
     DropPlan& D = ctx->plan;
-    std::vector<hipEvent_t> ev;
-    auto mark = [&]() {
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        (void)hipEventRecord(e, s);
-        ev.push_back(e);
-    };
+    std::vector<hipEvent_t> ev_rows, ev_cols;
+    std::vector<hipEvent_t>* er = timing ? &ev_rows : nullptr;
+    std::vector<hipEvent_t>* ec = timing ? &ev_cols : nullptr;
     const int G = ctx->drop_row_lanes, V = ctx->drop_col_voxels;
     const bool xcd = (D.flags & PCT_DROP_XCD_SLABS) != 0;
+    const bool bricks = (D.flags & PCT_DROP_BRICKS) != 0;
+    const uint32_t* lv = bricks ? D.vox_b : ctx->list_vox;
+    float* xx = bricks ? D.x_b : ctx->x;
     for (size_t bi = 0; bi < D.blocks.size(); bi++) {
         const DropPlan::Block& bk = D.blocks[bi];
         const int64_t hpw = kBlock / G;  // histories per workgroup
         uint32_t rper = 0;
-        const unsigned rgrid = slab_grid((bk.proc + hpw - 1) / hpw, xcd, &rper);
-        if (timing) mark();
-        dim3 rg(rgrid), rb(kBlock);
-#define PCT_ROWS(GG)                                                                                                  \
-    hipLaunchKernelGGL(k_drop_rows<GG>, rg, rb, 0, s, bk.start, bk.proc, rper, D.desc, ctx->list_vox, ctx->x, \
-                       P.lambda, D.r)
+        const dim3 rg(slab_grid((bk.proc + hpw - 1) / hpw, xcd, &rper)), blk(kBlock);
         switch (G) {
-        case 4: PCT_ROWS(4); break;
-        case 8: PCT_ROWS(8); break;
-        case 16: PCT_ROWS(16); break;
-        case 32: PCT_ROWS(32); break;
-        default: PCT_ROWS(64); break;
+        case 4: drop_launch(k_drop_rows<4>, rg, blk, s, er, bk.start, bk.proc, rper, D.desc, lv, xx, P.lambda, D.r); break;
+        case 8: drop_launch(k_drop_rows<8>, rg, blk, s, er, bk.start, bk.proc, rper, D.desc, lv, xx, P.lambda, D.r); break;
+        case 16: drop_launch(k_drop_rows<16>, rg, blk, s, er, bk.start, bk.proc, rper, D.desc, lv, xx, P.lambda, D.r); break;
+        case 32: drop_launch(k_drop_rows<32>, rg, blk, s, er, bk.start, bk.proc, rper, D.desc, lv, xx, P.lambda, D.r); break;
+        default: drop_launch(k_drop_rows<64>, rg, blk, s, er, bk.start, bk.proc, rper, D.desc, lv, xx, P.lambda, D.r); break;
         }
-#undef PCT_ROWS
-        if (timing) mark();
         const uint32_t U = bk.n_vox;
         if (bk.entries > 0 && U > 0) {
             uint32_t cper = 0;
             const int64_t vpw = (int64_t)kBlock * V;  // voxels per workgroup
-            const unsigned cgrid = slab_grid((U + vpw - 1) / vpw, xcd, &cper);
+            const dim3 cg(slab_grid((U + vpw - 1) / vpw, xcd, &cper));
             const uint32_t* t_vox = D.t_vox + bk.run_base;
             const uint32_t* t_off = D.t_off + bk.run_base;
             const uint32_t* t_rank = D.t_hist + bk.rank_base;
-            dim3 cg(cgrid), cbk(kBlock);
-#define PCT_COLS(A, VV) \
-    hipLaunchKernelGGL((k_drop_cols<A, VV>), cg, cbk, 0, s, U, cper, t_vox, t_off, t_rank, D.r, ctx->x)
+#define PCT_COLS(A, VV) drop_launch(k_drop_cols<A, VV>, cg, blk, s, ec, U, cper, t_vox, t_off, t_rank, D.r, xx)
 #define PCT_COLS_V(A)                  \
     switch (V) {                       \
     case 1: PCT_COLS(A, 1); break;     \
@@ -594,24 +675,25 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
 #undef PCT_COLS_V
 #undef PCT_COLS
         }
-        if (timing) {
-            mark();
-            timing->histories += bk.proc;
-        }
+        if (timing) timing->histories += bk.proc;
     }
     PCT_CHECK_HIP(ctx, hipGetLastError());
     if (timing) {
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-        for (size_t j = 0; j + 2 < ev.size(); j += 3) {
-            float a = 0, b = 0;
-            (void)hipEventElapsedTime(&a, ev[j], ev[j + 1]);
-            (void)hipEventElapsedTime(&b, ev[j + 1], ev[j + 2]);
+        for (size_t j = 0; j + 1 < ev_rows.size(); j += 2) {
+            float a = 0;
+            (void)hipEventElapsedTime(&a, ev_rows[j], ev_rows[j + 1]);
             timing->scatter_ms += a;
-            timing->apply_ms += b;
             timing->scatter_launches++;
+        }
+        for (size_t j = 0; j + 1 < ev_cols.size(); j += 2) {
+            float a = 0;
+            (void)hipEventElapsedTime(&a, ev_cols[j], ev_cols[j + 1]);
+            timing->apply_ms += a;
             timing->apply_launches++;
         }
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_rows) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_cols) (void)hipEventDestroy(e);
     }
     return PCT_OK;
 }

@@ -212,6 +212,7 @@ int pct_stage_fbp(pct_ctx* ctx)
     PCT_CHECK_HIP(ctx, hipGetLastError());
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
     pct_free(&d_cs);
+    PCT_TRY(pct_median_filter_fbp(ctx));  // FBP(), DS.cu:6150-6166 (x0 only)
     if (P.hull_type == PCT_HULL_FBP) PCT_TRY(pct_fbp_hull(ctx));
     ctx->stage = ST_FBP;
     return PCT_OK;

@@ -379,6 +379,7 @@ int pct_stage_select_hull(pct_ctx* ctx)
         return PCT_E_ARG;
     }
     PCT_CHECK_HIP(ctx, hipGetLastError());
+    PCT_TRY(pct_avg_filter_hull(ctx));  // hull_selection, DS.cu:7375-7386
     ctx->stage = ST_HULL_SEL;
     return PCT_OK;
 }

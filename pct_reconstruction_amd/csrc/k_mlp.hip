@@ -333,7 +333,8 @@ int pct_stage_endpoints(pct_ctx* ctx)
     H10 hp;
     for (int k = 0; k < 10; k++) hp.h[k] = ctx->h[k];
     if (n > 0)
-        hipLaunchKernelGGL(k_endpoints, dim3(grid_for(n)), dim3(kBlock), 0, s, n, g, ctx->hull, hp, ctx->first_voxel,
+        hipLaunchKernelGGL(k_endpoints, dim3(grid_for(n)), dim3(kBlock), 0, s, n, g,
+                           ctx->hull_endpoints ? ctx->hull_endpoints : ctx->hull, hp, ctx->first_voxel,
                            ctx->flag);
     PCT_CHECK_HIP(ctx, hipGetLastError());
     int64_t kept = 0;

@@ -68,6 +68,14 @@ extern "C" void pct_params_default(pct_params* p)
     p->quirk_a3_drop_floor = 1;
     p->quirk_voxel_decode = 1;
     p->drop_accum = PCT_ACCUM_FP64;
+    // filters: Configurations.h:154-168 (off, radius 0); threshold DS.h:483
+    p->median_filter_fbp = 0;
+    p->fbp_med_filter_radius = 0;
+    p->avg_filter_fbp = 0;
+    p->avg_filter_hull = 0;
+    p->hull_avg_filter_radius = 0;
+    p->reserved0 = 0;
+    p->hull_avg_filter_threshold = 0.1;
 }
 
 namespace {
@@ -102,6 +110,10 @@ const Key kKeys[] = {
     PK_D("SOURCE_RADIUS", source_radius), PK_D("RSP_AIR", rsp_air), PK_B("DATA_IN_MM", data_in_mm),
     PK_B("QUIRK_A1_BIN0", quirk_a1_bin0), PK_B("QUIRK_A3_DROP_FLOOR", quirk_a3_drop_floor),
     PK_B("QUIRK_VOXEL_DECODE", quirk_voxel_decode), PK_I("DROP_ACCUM", drop_accum),
+    PK_B("MEDIAN_FILTER_FBP", median_filter_fbp), PK_I("FBP_MED_FILTER_RADIUS", fbp_med_filter_radius),
+    PK_B("AVG_FILTER_FBP", avg_filter_fbp), PK_B("AVG_FILTER_HULL", avg_filter_hull),
+    PK_I("HULL_AVG_FILTER_RADIUS", hull_avg_filter_radius),
+    PK_D("HULL_AVG_FILTER_THRESHOLD", hull_avg_filter_threshold),
 };
 
 // Keys of settings.cfg that configure parts of the reference outside this path (I/O directories,
@@ -116,10 +128,10 @@ const char* kIgnored[] = {
     "HISTORIES_PER_THREAD", "VOXELS_PER_THREAD", "MAX_GPU_HISTORIES", "MAX_CUTS_HISTORIES", "MAX_ENDPOINTS_HISTORIES",
     "HULL_RSP_THRESHOLD", "EXPORT_PREPROCESSING", "IMPORT_PREPROCESSING", "PERFORM_RECONSTRUCTION",
     "PREPROCESS_OVERWRITE_OK", "RECON_OVERWRITE_OK", "MLP_IN_LOOP", "IMPORT_DATA_ITERATIVELY",
-    "IMPORT_MLP_ITERATIVELY", "TOTAL_DATA_2_GPU", "TOTAL_MLP_2_GPU", "MEDIAN_FILTER_HULL", "MEDIAN_FILTER_FBP",
-    "MEDIAN_FILTER_X_0", "MEDIAN_FILTER_X_K", "MEDIAN_FILTER_X", "AVG_FILTER_HULL", "AVG_FILTER_FBP",
-    "AVG_FILTER_X_0", "AVG_FILTER_X_K", "AVG_FILTER_X", "HULL_MED_FILTER_RADIUS", "FBP_MED_FILTER_RADIUS",
-    "X_0_MED_FILTER_RADIUS", "X_K_MED_FILTER_RADIUS", "X_MED_FILTER_RADIUS", "HULL_AVG_FILTER_RADIUS",
+    "IMPORT_MLP_ITERATIVELY", "TOTAL_DATA_2_GPU", "TOTAL_MLP_2_GPU", "MEDIAN_FILTER_HULL",
+    "MEDIAN_FILTER_X_0", "MEDIAN_FILTER_X_K", "MEDIAN_FILTER_X",
+    "AVG_FILTER_X_0", "AVG_FILTER_X_K", "AVG_FILTER_X", "HULL_MED_FILTER_RADIUS",
+    "X_0_MED_FILTER_RADIUS", "X_K_MED_FILTER_RADIUS", "X_MED_FILTER_RADIUS",
     "FBP_AVG_FILTER_RADIUS", "X_0_AVG_FILTER_RADIUS", "X_K_AVG_FILTER_RADIUS", "X_AVG_FILTER_RADIUS",
     "WRITE_MSC_COUNTS", "WRITE_SM_COUNTS", "WRITE_X_FBP", "WRITE_FBP_HULL", "WRITE_AVG_FBP", "WRITE_MEDIAN_FBP",
     "WRITE_BIN_WEPLS", "WRITE_WEPL_DISTS", "WRITE_SSD_ANGLES", "STDOUT_2_DISK", "USER_INPUT_REQUESTS_OFF",
@@ -251,5 +263,8 @@ extern "C" int pct_params_derive(pct_params* p, char* err, size_t err_len)
     if (p->hull_type < PCT_HULL_IMPORT || p->hull_type > PCT_HULL_FBP) return fail("HULL_TYPE out of range");
     if (p->x_0_type < PCT_X0_IMPORT || p->x_0_type > PCT_X0_ZEROS) return fail("X_0_TYPE out of range");
     if (p->fbp_filter_type < 0 || p->fbp_filter_type > 2) return fail("FBP_FILTER_TYPE out of range");
+    if (p->fbp_med_filter_radius < 0 || p->fbp_med_filter_radius > 10) return fail("FBP_MED_FILTER_RADIUS must be 0..10");
+    if (p->hull_avg_filter_radius < 0 || p->hull_avg_filter_radius > 64)
+        return fail("HULL_AVG_FILTER_RADIUS must be 0..64");
     return PCT_OK;
 }

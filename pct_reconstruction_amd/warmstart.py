@@ -52,9 +52,14 @@ def training_pair(seed: int, n: int, iterations: int = 40):
     return fbp, hull, target
 
 
-def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int) -> dict:
+def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int, long_iterations: int = 40) -> dict:
     P, r = _prepare(seed, n)
     try:
+        hull = r.image("hull").astype(bool)
+        # reference image: `long_iterations` DROP iterations from the FBP start
+        r.reset_iterate()
+        r.DROP(long_iterations)
+        x_long = r.image("x")
         # FBP start
         r.reset_iterate()
         r.synchronize()
@@ -85,11 +90,17 @@ def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int) -> dict:
         t_net = time.perf_counter() - t0
     finally:
         r.close()
+
+    def rel(a, b):  # relative L2 difference inside the hull (DROP never changes voxels outside it)
+        return float(np.linalg.norm((a - b)[hull]) / np.linalg.norm(b[hull]))
+
     return {"seed": seed, "used_net": judge["used_net"], "residual_x0_fbp": r0_fbp,
             "residual_x0_net": judge["residual_net"], "residual_target": r_ref, "residual_reached": res,
             "iterations_fbp": ref_iterations, "iterations_net": k, "t_fbp_s": t_fbp, "t_net_s": t_net,
             "speedup": t_fbp / t_net if t_net > 0 else None,
-            "image_rel_diff": float(np.linalg.norm(x_net - x_fbp) / np.linalg.norm(x_fbp))}
+            "hull_rel_diff_net_vs_fbp": rel(x_net, x_fbp),
+            f"hull_rel_err_fbp_vs_{long_iterations}it": rel(x_fbp, x_long),
+            f"hull_rel_err_net_vs_{long_iterations}it": rel(x_net, x_long)}
 
 
 def main(argv=None) -> int:

@@ -67,6 +67,10 @@ def _run_and_compare(P, data, oracle, case):
         print(f"[{case}] fbp mismatches {n_fbp} max abs {np.max(np.abs(fbp - ref['fbp']))}", flush=True)
         assert n_fbp == 0
         assert _mismatch(r.image("hull").reshape(-1), ref["hull"].astype(np.float32)) == 0
+        if "fbp_median" in ref:
+            assert _mismatch(r.image("fbp_median").reshape(-1), ref["fbp_median"]) == 0
+        if "hull_filtered" in ref:
+            assert _mismatch(r.image("hull_filtered").reshape(-1), ref["hull_filtered"].astype(np.float32)) == 0
         assert _mismatch(r.image("x0").reshape(-1), ref["x0"]) == 0
         r.reconstruction_cuts()
         assert r.count("reconstruction") == ref["h_recon"].shape[1]
@@ -94,16 +98,17 @@ def _run_and_compare(P, data, oracle, case):
         assert _mismatch(r.image("x").reshape(-1), x) == 0
         # scheduling (visiting order, XCD slabs, voxels per lane) never changes a bit; the lanes per
         # history only regroup an FP64 sum of floats
-        for lanes, vox, spatial, slabs in [(16, 1, False, False), (16, 8, True, False), (16, 2, False, True),
-                                           (64, 4, True, True), (4, 4, True, True)]:
-            r.set_launch(lanes, vox, spatial, slabs)
+        for lanes, vox, spatial, slabs, bricks in [(16, 1, False, False, False), (16, 8, True, False, True),
+                                                   (16, 2, False, True, False), (16, 1, True, True, True),
+                                                   (64, 4, True, True, False), (4, 4, True, True, True)]:
+            r.set_launch(lanes, vox, spatial, slabs, bricks)
             r.reset_iterate()
             r.DROP(P.iterations)
             xs = r.image("x").reshape(-1)
             if lanes == 16:
-                assert _mismatch(xs, x) == 0, (lanes, vox, spatial, slabs)
+                assert _mismatch(xs, x) == 0, (lanes, vox, spatial, slabs, bricks)
             else:
-                assert np.linalg.norm(xs - x) <= 1e-7 * np.linalg.norm(x), (lanes, vox, spatial, slabs)
+                assert np.linalg.norm(xs - x) <= 1e-7 * np.linalg.norm(x), (lanes, vox, spatial, slabs, bricks)
         r.set_launch()
         r.reset_iterate()
         r.DROP(P.iterations)
@@ -121,6 +126,12 @@ def test_parity_cylinder_sc(oracle):
 def test_parity_head_msc(oracle):
     P, data = small_head()
     _run_and_compare(P, data, oracle, "head-MSC")
+
+
+def test_parity_head_with_fbp_median_and_hull_average_filters(oracle):
+    P, data = small_head(n=50000, seed=11, median_filter_fbp=1, fbp_med_filter_radius=2, avg_filter_hull=1,
+                         hull_avg_filter_radius=1, hull_avg_filter_threshold=0.3, x_0_type=3)
+    _run_and_compare(P, data, oracle, "head-filters")
 
 
 def test_drop_transposition_groups_do_not_change_bits(monkeypatch):

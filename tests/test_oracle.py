@@ -334,6 +334,34 @@ def test_endpoints_on_the_cylinder_hull(oracle):
     assert (first0 == P.num_voxels).all()
 
 
+def test_median_filter_matches_library_median_inside_and_zero_border(oracle):
+    from scipy import ndimage
+    P = _params(columns=20, rows=14, slice_thickness=2.0)
+    rng = np.random.default_rng(8)
+    img = rng.standard_normal((P.slices, 14, 20)).astype(np.float32)
+    img[:, 5, 5:9] = 3.0  # ties
+    for r in (1, 2, 3):
+        out = oracle.median_filter_2d(P, img.reshape(-1), r).reshape(img.shape)
+        ref = ndimage.median_filter(img, size=(1, 2 * r + 1, 2 * r + 1), mode="nearest")
+        inner = np.zeros(img.shape, bool)
+        inner[:, r:14 - r, r:20 - r] = True
+        assert np.array_equal(out[inner], ref[inner]), r
+        assert (out[~inner] == 0).all(), r
+
+
+def test_hull_average_filter_matches_window_sums(oracle):
+    from scipy import ndimage
+    P = _params(columns=18, rows=16, slice_thickness=2.0)
+    rng = np.random.default_rng(9)
+    hull = (rng.random((P.slices, 16, 18)) < 0.5).astype(np.uint8)
+    for r, thr in ((1, 0.1), (2, 0.5), (3, 0.9)):
+        out = oracle.avg_filter_hull(P, hull.reshape(-1), r, thr).reshape(hull.shape)
+        k = np.ones((1, 2 * r + 1, 2 * r + 1))
+        s = ndimage.correlate(hull.astype(np.float64), k, mode="constant")            # clipped window sum
+        n = ndimage.correlate(np.ones(hull.shape), k, mode="constant")                # clipped window size
+        assert np.array_equal(out, (s > n * thr).astype(np.uint8)), (r, thr)
+
+
 # ---------------------------------------------------------------- FBP
 def test_backprojection_of_constant_sinogram_at_centre(oracle):
     P = _params(columns=41, rows=41, slice_thickness=8.0, recon_cyl_radius=10.0)
