@@ -44,8 +44,9 @@ enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1, PCT_ACCUM_KAHAN32 
  * gathers along neighbouring parallel paths; XCD_SLABS gives each of the 8 XCDs a contiguous slab
  * of those histories (row kernel) and of the touched voxels (column kernel); BRICKS keeps the image
  * in 4x4x2-voxel bricks (one 128-byte line each) during DROP, so consecutive voxels of a path and
- * neighbouring paths share cache lines. */
-enum pct_drop_flags { PCT_DROP_SPATIAL_ORDER = 1, PCT_DROP_XCD_SLABS = 2, PCT_DROP_BRICKS = 4 };
+ * neighbouring paths share cache lines; COLS_LDS runs the per-voxel sums with one wavefront per 64
+ * voxels that stages their entries' updates in LDS with coalesced loads. */
+enum pct_drop_flags { PCT_DROP_SPATIAL_ORDER = 1, PCT_DROP_XCD_SLABS = 2, PCT_DROP_BRICKS = 4, PCT_DROP_COLS_LDS = 8 };
 
 /*
  * Parameters.  Field names follow the settings.cfg keys (settings.cfg:1-164) and the

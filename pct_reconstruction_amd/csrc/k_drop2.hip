@@ -288,6 +288,61 @@ __global__ void __launch_bounds__(kBlock) k_drop_cols(uint32_t U, uint32_t per, 
         if (e[q] > b[q]) x[v[q]] = xv[q] + col_update<ACC, K>(b[q], e[q], rv[q], t_rank, r);
 }
 
+// Column update with the entries staged per wavefront (PCT_DROP_COLS_LDS).  A one-wave workgroup
+// takes 64 consecutive touched voxels; their entries are one contiguous range of t_rank.  All 64
+// lanes load that range coalesced and gather the updates r into LDS (kStage entries per round);
+// then each lane sums its own voxel's segment from LDS in history order, exactly as col_update.
+constexpr int kStage = 1024;
+
+template <int ACC>
+__global__ void __launch_bounds__(kWave) k_drop_cols_lds(uint32_t U, const uint32_t* __restrict__ t_vox,
+                                                          const uint32_t* __restrict__ t_off,
+                                                          const uint32_t* __restrict__ t_rank,
+                                                          const double* __restrict__ r, float* __restrict__ x)
+{
+    __shared__ double stage[kStage];
+    const int lane = threadIdx.x;
+    const uint32_t u0 = blockIdx.x * kWave;
+    const uint32_t u = u0 + lane;
+    const bool ok = u < U;
+    const uint32_t b = ok ? t_off[u] : 0u, e = ok ? t_off[u + 1] : 0u;
+    const uint32_t v = ok ? t_vox[u] : 0u;
+    const float xv = ok && e > b ? x[v] : 0.0f;
+    const uint32_t last = u0 + kWave < U ? u0 + kWave : U;
+    const uint32_t E0 = t_off[u0], E1 = t_off[last];
+    double s64 = 0.0;
+    float s32 = 0.0f, c32 = 0.0f;
+    bool first = true;
+    for (uint32_t R0 = E0; R0 < E1; R0 += kStage) {
+        const uint32_t R1 = R0 + kStage < E1 ? R0 + kStage : E1;
+        for (uint32_t j = R0 + lane; j < R1; j += kWave) stage[j - R0] = r[t_rank[j]];
+        __syncthreads();
+        const uint32_t j0 = b > R0 ? b : R0, j1 = e < R1 ? e : R1;
+        for (uint32_t j = j0; j < j1; j++) {
+            const double rv = stage[j - R0];
+            if (ACC == PCT_ACCUM_FP64) {
+                s64 = first ? rv : s64 + rv;
+            } else if (ACC == PCT_ACCUM_FP32) {
+                s32 = first ? (float)rv : s32 + (float)rv;
+            } else {
+                const float y = (float)rv - c32;
+                const float t = s32 + y;
+                c32 = (t - s32) - y;
+                s32 = t;
+            }
+            first = false;
+        }
+        __syncthreads();
+    }
+    if (ok && e > b) {
+        const uint32_t n = e - b;
+        float upd;
+        if (ACC == PCT_ACCUM_FP64) upd = (float)(s64 / (double)n);
+        else upd = s32 / (float)n;
+        x[v] = xv + upd;
+    }
+}
+
 inline unsigned grid_for(int64_t n, int block = kBlock)
 {
     int64_t g = (n + block - 1) / block;
@@ -665,7 +720,15 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
     case 8: PCT_COLS(A, 8); break;     \
     default: PCT_COLS(A, 4); break;    \
     }
-            if (P.drop_accum == PCT_ACCUM_FP64) {
+            const dim3 lg((U + kWave - 1) / kWave), lb(kWave);
+            if (D.flags & PCT_DROP_COLS_LDS) {
+                if (P.drop_accum == PCT_ACCUM_FP64)
+                    drop_launch(k_drop_cols_lds<PCT_ACCUM_FP64>, lg, lb, s, ec, U, t_vox, t_off, t_rank, D.r, xx);
+                else if (P.drop_accum == PCT_ACCUM_FP32)
+                    drop_launch(k_drop_cols_lds<PCT_ACCUM_FP32>, lg, lb, s, ec, U, t_vox, t_off, t_rank, D.r, xx);
+                else
+                    drop_launch(k_drop_cols_lds<PCT_ACCUM_KAHAN32>, lg, lb, s, ec, U, t_vox, t_off, t_rank, D.r, xx);
+            } else if (P.drop_accum == PCT_ACCUM_FP64) {
                 PCT_COLS_V(PCT_ACCUM_FP64)
             } else if (P.drop_accum == PCT_ACCUM_FP32) {
                 PCT_COLS_V(PCT_ACCUM_FP32)

@@ -98,17 +98,18 @@ def _run_and_compare(P, data, oracle, case):
         assert _mismatch(r.image("x").reshape(-1), x) == 0
         # scheduling (visiting order, XCD slabs, voxels per lane) never changes a bit; the lanes per
         # history only regroup an FP64 sum of floats
-        for lanes, vox, spatial, slabs, bricks in [(16, 1, False, False, False), (16, 8, True, False, True),
-                                                   (16, 2, False, True, False), (16, 1, True, True, True),
-                                                   (64, 4, True, True, False), (4, 4, True, True, True)]:
-            r.set_launch(lanes, vox, spatial, slabs, bricks)
+        for shape in [(16, 1, False, False, False, False), (16, 8, True, False, True, False),
+                      (16, 2, False, True, False, True), (16, 1, True, True, True, True),
+                      (16, 1, True, False, True, True), (64, 4, True, True, False, False),
+                      (4, 4, True, True, True, True)]:
+            r.set_launch(*shape)
             r.reset_iterate()
             r.DROP(P.iterations)
             xs = r.image("x").reshape(-1)
-            if lanes == 16:
-                assert _mismatch(xs, x) == 0, (lanes, vox, spatial, slabs, bricks)
+            if shape[0] == 16:
+                assert _mismatch(xs, x) == 0, shape
             else:
-                assert np.linalg.norm(xs - x) <= 1e-7 * np.linalg.norm(x), (lanes, vox, spatial, slabs, bricks)
+                assert np.linalg.norm(xs - x) <= 1e-7 * np.linalg.norm(x), shape
         r.set_launch()
         r.reset_iterate()
         r.DROP(P.iterations)
