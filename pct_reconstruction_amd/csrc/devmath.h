@@ -55,3 +55,15 @@ struct DevGeom {
     double X0, Y0, Z0, VW, VH, VT;
     double Rc, H, PI;
 };
+
+// Brick index used by DROP: voxel (x, y, z) of the C x R x S grid -> brick (z/2, y/4, x/4) of a grid
+// of nbx x nby brick columns, times 32, plus (z%2, y%4, x%4) within the brick (32 floats = 128 B).
+struct BrickGeom {
+    uint32_t C, R, nbx, nby;
+};
+PCT_HD uint32_t brick_of(uint32_t v, BrickGeom g)
+{
+    const uint32_t x = v % g.C, t = v / g.C, y = t % g.R, z = t / g.R;
+    const uint32_t b = ((z >> 1) * g.nby + (y >> 2)) * g.nbx + (x >> 2);
+    return b * 32u + (z & 1u) * 16u + (y & 3u) * 4u + (x & 3u);
+}

@@ -91,6 +91,7 @@ struct DropPlan {
     uint32_t* vox_b = nullptr;     // list entries as brick indices (same CSR layout as list_vox)
     float* x_b = nullptr;          // image in brick order during DROP
     int64_t cap_vox_b = 0, cap_x_b = 0, nv_b = 0;
+    bool vox_b_ready = false;      // vox_b holds the current lists (written by the MLP kernel)
     int nbx = 0, nby = 0;
     int64_t cap_entries = 0, cap_off = 0, cap_r = 0, cap_perm = 0, cap_group_e = 0, cap_group_h = 0,
             cap_group_b = 0;

@@ -44,18 +44,6 @@ __device__ __forceinline__ uint32_t slab_index(uint32_t per)
     return per ? (blockIdx.x % kXcds) * per + blockIdx.x / kXcds : blockIdx.x;
 }
 
-// Brick layout: voxel (x, y, z) of the C x R x S grid -> brick ((z/2, y/4, x/4) in a grid of
-// nbx x nby columns of bricks) * 32 + (z%2, y%4, x%4).  32 floats = one 128-byte line.
-struct BrickGeom {
-    uint32_t C, R, nbx, nby;
-};
-__device__ __forceinline__ uint32_t brick_of(uint32_t v, BrickGeom g)
-{
-    const uint32_t x = v % g.C, t = v / g.C, y = t % g.R, z = t / g.R;
-    const uint32_t b = ((z >> 1) * g.nby + (y >> 2)) * g.nbx + (x >> 2);
-    return b * 32u + (z & 1u) * 16u + (y & 3u) * 4u + (x & 3u);
-}
-
 // list entries (all `n` of the CSR capacity; unused slots hold no voxel and are mapped harmlessly)
 __global__ void k_bricks_list(int64_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, BrickGeom g,
                               uint32_t nv)
@@ -74,6 +62,28 @@ __global__ void k_x_bricks(int64_t nv, float* __restrict__ x, float* __restrict_
     const uint32_t b = brick_of((uint32_t)v, g);
     if (to_bricks) xb[b] = x[v];
     else x[v] = xb[b];
+}
+
+// entries of every raw block: sum of the counts of its processed histories with at least
+// min_len voxels (one workgroup per block)
+__global__ void __launch_bounds__(256) k_block_entries(int64_t BS, const int32_t* __restrict__ proc_raw,
+                                                        const uint32_t* __restrict__ cnt, int min_len,
+                                                        int64_t* __restrict__ out)
+{
+    __shared__ int64_t part[256];
+    const int64_t rb = blockIdx.x, start = rb * BS, proc = proc_raw[rb];
+    int64_t s = 0;
+    for (int64_t i = threadIdx.x; i < proc; i += blockDim.x) {
+        const uint32_t c = cnt[start + i];
+        if ((int)c >= min_len) s += c;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[rb] = part[0];
 }
 
 // ---- transposition of a group of blocks (once per list build) ----
@@ -209,45 +219,56 @@ __global__ void __launch_bounds__(kBlock) k_drop_rows(int64_t start, int64_t pro
     }
 }
 
+// Running per-voxel sum in history order: FP64 (s = u_1 + u_2 + ...), FP32 (the reference's float
+// sums), or FP32 with Kahan compensation (from s = c = 0).
+template <int ACC>
+struct ColAcc {
+    double s64 = 0.0;
+    float s32 = 0.0f, c32 = 0.0f;
+    bool first = true;
+    __device__ __forceinline__ void add(double rv)
+    {
+        if (ACC == PCT_ACCUM_FP64) {
+            s64 = first ? rv : s64 + rv;
+        } else if (ACC == PCT_ACCUM_FP32) {
+            s32 = first ? (float)rv : s32 + (float)rv;
+        } else {
+            const float y = (float)rv - c32;
+            const float t = s32 + y;
+            c32 = (t - s32) - y;
+            s32 = t;
+        }
+        first = false;
+    }
+    __device__ __forceinline__ float mean(uint32_t n) const
+    {
+        return ACC == PCT_ACCUM_FP64 ? (float)(s64 / (double)n) : s32 / (float)n;
+    }
+};
+
 // Sum of the n = e - b updates of one voxel in history order, divided by n (the voxel's S).
-// The first min(n, K) updates arrive prefetched in rv; the rest are loaded here.
+// The first min(n, K) updates arrive prefetched in rv; the rest are loaded K at a time (K rank
+// loads, then K gathers) so that a long column costs two memory round trips per K entries.
 template <int ACC, int K>
 __device__ __forceinline__ float col_update(uint32_t b, uint32_t e, const double* rv,
                                             const uint32_t* __restrict__ t_rank, const double* __restrict__ r)
 {
-    const uint32_t n = e - b;
-    if (ACC == PCT_ACCUM_FP64) {
-        double s = rv[0];
+    ColAcc<ACC> a;
 #pragma unroll
-        for (int m = 1; m < K; m++)
-            if ((uint32_t)m < n) s += rv[m];
-        for (uint32_t k = b + K; k < e; k++) s += r[t_rank[k]];
-        return (float)(s / (double)n);
-    } else if (ACC == PCT_ACCUM_FP32) {
-        float s = (float)rv[0];
+    for (int m = 0; m < K; m++)
+        if (b + m < e) a.add(rv[m]);
+    for (uint32_t c0 = b + K; c0 < e; c0 += K) {
+        uint32_t rk[K];
+        double rw[K];
 #pragma unroll
-        for (int m = 1; m < K; m++)
-            if ((uint32_t)m < n) s += (float)rv[m];
-        for (uint32_t k = b + K; k < e; k++) s += (float)r[t_rank[k]];
-        return s / (float)n;
-    } else {  // Kahan-compensated FP32
-        float s = 0.0f, c = 0.0f, y, t;
+        for (int m = 0; m < K; m++) rk[m] = c0 + m < e ? t_rank[c0 + m] : 0u;
+#pragma unroll
+        for (int m = 0; m < K; m++) rw[m] = c0 + m < e ? r[rk[m]] : 0.0;
 #pragma unroll
         for (int m = 0; m < K; m++)
-            if ((uint32_t)m < n) {
-                y = (float)rv[m] - c;
-                t = s + y;
-                c = (t - s) - y;
-                s = t;
-            }
-        for (uint32_t k = b + K; k < e; k++) {
-            y = (float)r[t_rank[k]] - c;
-            t = s + y;
-            c = (t - s) - y;
-            s = t;
-        }
-        return s / (float)n;
+            if (c0 + m < e) a.add(rw[m]);
     }
+    return a.mean(e - b);
 }
 
 // per touched voxel: image_update_GPU (DS.cu:9398-9419) with the block's sums formed in history
@@ -310,37 +331,16 @@ __global__ void __launch_bounds__(kWave) k_drop_cols_lds(uint32_t U, const uint3
     const float xv = ok && e > b ? x[v] : 0.0f;
     const uint32_t last = u0 + kWave < U ? u0 + kWave : U;
     const uint32_t E0 = t_off[u0], E1 = t_off[last];
-    double s64 = 0.0;
-    float s32 = 0.0f, c32 = 0.0f;
-    bool first = true;
+    ColAcc<ACC> a;
     for (uint32_t R0 = E0; R0 < E1; R0 += kStage) {
         const uint32_t R1 = R0 + kStage < E1 ? R0 + kStage : E1;
         for (uint32_t j = R0 + lane; j < R1; j += kWave) stage[j - R0] = r[t_rank[j]];
         __syncthreads();
         const uint32_t j0 = b > R0 ? b : R0, j1 = e < R1 ? e : R1;
-        for (uint32_t j = j0; j < j1; j++) {
-            const double rv = stage[j - R0];
-            if (ACC == PCT_ACCUM_FP64) {
-                s64 = first ? rv : s64 + rv;
-            } else if (ACC == PCT_ACCUM_FP32) {
-                s32 = first ? (float)rv : s32 + (float)rv;
-            } else {
-                const float y = (float)rv - c32;
-                const float t = s32 + y;
-                c32 = (t - s32) - y;
-                s32 = t;
-            }
-            first = false;
-        }
+        for (uint32_t j = j0; j < j1; j++) a.add(stage[j - R0]);
         __syncthreads();
     }
-    if (ok && e > b) {
-        const uint32_t n = e - b;
-        float upd;
-        if (ACC == PCT_ACCUM_FP64) upd = (float)(s64 / (double)n);
-        else upd = s32 / (float)n;
-        x[v] = xv + upd;
-    }
+    if (ok && e > b) x[v] = xv + a.mean(e - b);
 }
 
 inline unsigned grid_for(int64_t n, int block = kBlock)
@@ -410,6 +410,7 @@ void pct_drop_plan_free(DropPlan* D)
     pct_free(&D->vox_b);
     pct_free(&D->x_b);
     D->cap_vox_b = D->cap_x_b = 0;
+    D->vox_b_ready = false;
     D->cap_entries = D->cap_off = D->cap_r = D->cap_perm = D->cap_group_e = D->cap_group_h = D->cap_group_b = 0;
     D->ready = false;
 }
@@ -448,22 +449,36 @@ int pct_drop_prepare(pct_ctx* ctx)
     const int64_t max_group_blocks = 1ll << block_bits;
     const int64_t kGroupEntries = ctx->drop_group_entries, kGroupHistories = 1ll << 30;
 
-    // block ranges and entry counts (host pass over the list counts)
-    std::vector<uint32_t> cnt(n > 0 ? n : 1);
-    if (n > 0) PCT_CHECK_HIP(ctx, hipMemcpy(cnt.data(), ctx->list_cnt, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    // block ranges (DS.cu:9558 floors the processed count) and their entry counts (on the device)
     const int64_t n_raw = (n + BS - 1) / BS;
     std::vector<int32_t> proc_raw(n_raw > 0 ? n_raw : 1, 0);
+    for (int64_t rb = 0; rb < n_raw; rb++) {
+        const int64_t m = n - rb * BS < BS ? n - rb * BS : BS;
+        const int64_t proc = P.quirk_a3_drop_floor ? (m / P.histories_per_block) * P.histories_per_block : m;
+        proc_raw[rb] = proc > 0 ? (int32_t)proc : 0;
+    }
+    int32_t* proc_dev = nullptr;
+    int64_t* e_dev = nullptr;
+    PCT_TRY(pct_alloc(ctx, &proc_dev, n_raw + 1));
+    PCT_TRY(pct_alloc(ctx, &e_dev, n_raw + 1));
+    std::vector<int64_t> e_raw(n_raw > 0 ? n_raw : 1, 0);
+    if (n_raw > 0) {
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(proc_dev, proc_raw.data(), n_raw * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_block_entries, dim3((unsigned)n_raw), dim3(256), 0, s, BS, proc_dev, ctx->list_cnt,
+                           P.min_mlp_length, e_dev);
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(e_raw.data(), e_dev, n_raw * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    }
+    pct_free(&e_dev);
     int64_t total = 0, max_proc = 0;
     for (int64_t rb = 0; rb < n_raw; rb++) {
         const int64_t start = rb * BS;
         const int64_t m = n - start < BS ? n - start : BS;
-        const int64_t proc = P.quirk_a3_drop_floor ? (m / P.histories_per_block) * P.histories_per_block : m;
+        const int64_t proc = proc_raw[rb];
         if (proc <= 0) continue;
-        proc_raw[rb] = (int32_t)proc;
-        int64_t e = 0;
-        for (int64_t i = start; i < start + proc; i++)
-            if ((int)cnt[i] >= P.min_mlp_length) e += cnt[i];
+        const int64_t e = e_raw[rb];
         if (e >= (1ll << 30)) {  // a group holds at least one whole block
+            pct_free(&proc_dev);
             ctx->err = "DROP block with 2^30 or more list entries";
             return PCT_E_LIMIT;
         }
@@ -533,7 +548,7 @@ int pct_drop_prepare(pct_ctx* ctx)
     }
     const uint32_t* lvox = ctx->list_vox;
     if (bricks) {
-        if (D.cap_vox_b < ctx->list_cap + 1 || !D.vox_b) {
+        if (!D.vox_b_ready && (D.cap_vox_b < ctx->list_cap + 1 || !D.vox_b)) {
             PCT_TRY(pct_alloc(ctx, &D.vox_b, ctx->list_cap + 1));
             D.cap_vox_b = ctx->list_cap + 1;
         }
@@ -543,14 +558,12 @@ int pct_drop_prepare(pct_ctx* ctx)
             PCT_CHECK_HIP(ctx, hipMemsetAsync(D.x_b, 0, D.nv_b * sizeof(float), s));  // padding stays 0
         }
         const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
-        if (ctx->list_cap > 0)
+        if (!D.vox_b_ready && ctx->list_cap > 0)  // lists built without the brick index
             hipLaunchKernelGGL(k_bricks_list, dim3(grid_for(ctx->list_cap)), dim3(kBlock), 0, s, ctx->list_cap,
                                ctx->list_vox, D.vox_b, bg, (uint32_t)P.num_voxels);
+        D.vox_b_ready = true;
         lvox = D.vox_b;
     }
-    int32_t* proc_dev = nullptr;
-    PCT_TRY(pct_alloc(ctx, &proc_dev, n_raw + 1));
-    PCT_CHECK_HIP(ctx, hipMemcpyAsync(proc_dev, proc_raw.data(), n_raw * sizeof(int32_t), hipMemcpyHostToDevice, s));
     // library scratch size for the largest group
     {
         size_t need = 0, t = 0;

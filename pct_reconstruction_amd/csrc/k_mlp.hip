@@ -431,6 +431,7 @@ int pct_stage_build_lists(pct_ctx* ctx)
         PCT_TRY(pct_alloc(ctx, &ctx->list_vox, total));
         ctx->list_cap = total;
     }
+    ctx->plan.vox_b_ready = false;  // the brick copy of the lists belongs to the previous build
     if (n > 0) {
         size_t lds = (size_t)a.n_steps * 6 * sizeof(double);
         if (lds > 64 * 1024) {
