@@ -104,6 +104,19 @@ typedef struct pct_params {
     int32_t hull_avg_filter_radius;  /* HULL_AVG_FILTER_RADIUS */
     int32_t reserved0;
     double hull_avg_filter_threshold; /* HULL_AVG_FILTER_THRESHOLD (DS.h:483, 0.1) */
+    /* total variation superiorization between DROP iterations (DROP_GPU DS.cu:9889-9915, NTVS_iteration
+     * :10583, constants DS.h:427-444).  Compiled on in the reference; off here by default (the plain method). */
+    int32_t tvs_on;                  /* TVS_ON */
+    int32_t tvs_first;               /* TVS_FIRST: 1 = before each DROP iteration, 0 = after */
+    int32_t tvs_conditioned;         /* TVS_CONDITIONED: shrink BETA until TV does not grow (:10476) */
+    int32_t tvs_old;                 /* TVS_OLD: one repetition and no random draw of L (:9890-9892) */
+    int32_t tvs_repetitions;         /* TVS_REPETITIONS (DS.h:443, 5) */
+    int32_t tvs_l0;                  /* L_0 (DS.h:437, 0) */
+    int32_t tvs_seed;                /* seed of the draw of L (the reference seeds from the clock, :11480) */
+    int32_t reserved1;
+    double tvs_a;                    /* A (DS.h:436, 0.5): BETA = A^L */
+    double tvs_min_eta;              /* TVS_MIN_ETA (DS.h:434, 1e-40): length of the A^L table */
+    double tv_threshold;             /* TV_THRESHOLD: the reference's (1/1000) is integer division, 0 */
 
     /* ---- derived (pct_params_derive) ---- */
     int32_t t_bins, v_bins, angular_bins, num_bins;
@@ -192,6 +205,16 @@ int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_voxels, int dr
  * pct_drop_iterate. */
 int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, double* cols_ms, int64_t* launches,
                      int64_t* histories, int64_t* gathered, int64_t* scattered, int64_t* touched);
+/* Total variation superiorization record of the DROP iterations run since the initial iterate (or
+ * pct_set_image / pct_reset_iterate): for each NTVS_iteration (DS.cu:10583) the total variation of x
+ * before and after it (the TV_x_values pushes of :10587 and :10596) and the exponent L after it.
+ * *count receives the number of records; at most `capacity` are copied (arrays may be NULL). */
+int pct_get_tvs_log(pct_ctx* ctx, size_t capacity, double* tv_before, double* tv_after, int32_t* l_after,
+                    int64_t* count);
+/* Total variation of the current iterate (calculate_total_variation, DS.cu:10165): the float terms
+ * sqrt(dy^2 + dx^2) over rows < ROWS-1, columns < COLUMNS-1 of every slice, summed in FP64 in the
+ * library's fixed order (tiles of 1024 voxels). */
+int pct_total_variation(pct_ctx* ctx, double* tv);
 
 /* ---------------------------------------------------------------- outputs ---- */
 enum pct_image_kind { PCT_IMG_X = 0, PCT_IMG_FBP = 1, PCT_IMG_X0 = 2, PCT_IMG_HULL = 3,

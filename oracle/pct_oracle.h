@@ -66,6 +66,17 @@ void o_fbp_hull(const pct_params* p, const float* image, uint8_t* hull);
 void o_median_filter_2d(const pct_params* p, const float* in, int radius, float* out);
 void o_avg_filter_hull(const pct_params* p, const uint8_t* in, int radius, double threshold, uint8_t* out);
 
+/* Total variation superiorization (pct_tvs_oracle.c; DS.cu:10149-10602).  mode 0 = the reference's
+ * serial float sums, 1 = the library's tiled FP64 sums. */
+double o_tile_sum(const double* t, int64_t n);
+int o_tvs_eta_len(const pct_params* p);
+float o_tvs_eta(const pct_params* p, int L);
+int o_tvs_draw(int seed, int k, int lo, int hi);
+float o_tv(const pct_params* p, const float* img, int mode);
+float o_tvs_perturbation(const pct_params* p, const float* img, float* G, float* v, int mode);
+void o_tvs_iteration(const pct_params* p, const uint8_t* hull, float* x, int k, int* L, int mode,
+                     float* tv_before, float* tv_after);
+
 /* MLP lookup tables (:8421-8502), FP64. */
 typedef struct o_tables {
     int64_t n_trig, n_coeff, n_poly;

@@ -56,6 +56,13 @@ def lib() -> C.CDLL:
         "o_drop_iteration": ([P(Params), i64, vp, vp, vp, vp, vp, C.c_int, C.c_int], None),
         "o_drop_iteration_fp64": ([P(Params), i64, vp, vp, vp, vp, vp], None),
         "o_residual": ([i64, vp, vp, vp, vp, vp, C.c_int], C.c_double),
+        "o_tile_sum": ([vp, i64], C.c_double),
+        "o_tvs_eta_len": ([P(Params)], C.c_int),
+        "o_tvs_eta": ([P(Params), C.c_int], C.c_float),
+        "o_tvs_draw": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "o_tv": ([P(Params), vp, C.c_int], C.c_float),
+        "o_tvs_perturbation": ([P(Params), vp, vp, vp, C.c_int], C.c_float),
+        "o_tvs_iteration": ([P(Params), vp, vp, C.c_int, P(C.c_int), C.c_int, P(C.c_float), P(C.c_float)], None),
         "on_num_params": ([], i64),
         "on_loss_grad": ([C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp], C.c_double),
     }
@@ -257,6 +264,53 @@ def residual(off, vox, chord, wepl, x, min_len):
                             _p(np.ascontiguousarray(x, np.float32).reshape(-1)), int(min_len))
 
 
+TVS_SUM_SERIAL_FLOAT, TVS_SUM_TILED_FP64 = 0, 1
+
+
+def tile_sum(terms):
+    """FP64 sum in the library's fixed order (tiles of 1024, halving trees)."""
+    t = np.ascontiguousarray(terms, np.float64)
+    return lib().o_tile_sum(_p(t), t.size)
+
+
+def tvs_eta(P, L):
+    """TVS_eta_sequence_h[L - L_0] (DS.cu:10149-10156), 0 past the table."""
+    return lib().o_tvs_eta(C.byref(P), int(L))
+
+
+def tvs_eta_len(P):
+    return lib().o_tvs_eta_len(C.byref(P))
+
+
+def tvs_draw(seed, k, lo, hi):
+    return lib().o_tvs_draw(int(seed), int(k), int(lo), int(hi))
+
+
+def total_variation(P, img, mode=TVS_SUM_TILED_FP64):
+    """calculate_total_variation (DS.cu:10165) with the given summation order."""
+    img = np.ascontiguousarray(img, np.float32)
+    return float(lib().o_tv(C.byref(P), _p(img), int(mode)))
+
+
+def tvs_perturbation(P, img, mode=TVS_SUM_TILED_FP64):
+    """generate_perturbation_array (DS.cu:10262): returns (G, v, norm)."""
+    img = np.ascontiguousarray(img, np.float32)
+    G = np.empty(P.num_voxels, np.float32)
+    v = np.empty(P.num_voxels, np.float32)
+    n = lib().o_tvs_perturbation(C.byref(P), _p(img), _p(G), _p(v), int(mode))
+    return G, v, float(n)
+
+
+def tvs_iteration(P, hull, x, k, L, mode=TVS_SUM_TILED_FP64):
+    """NTVS_iteration (DS.cu:10583) for DROP iteration k (1-based).  Returns (x, L, tv_before, tv_after)."""
+    x = np.ascontiguousarray(x, np.float32).copy()
+    hull = np.ascontiguousarray(hull, np.uint8)
+    Lc = C.c_int(int(L))
+    tb, ta = C.c_float(), C.c_float()
+    lib().o_tvs_iteration(C.byref(P), _p(hull), _p(x), int(k), C.byref(Lc), int(mode), C.byref(tb), C.byref(ta))
+    return x, Lc.value, float(tb.value), float(ta.value)
+
+
 def convert_units(P, t, v, u, wepl):
     """convert_mm_2_cm (DS.cu:4070) and apply_tuv_shifts (DS.cu:4090) in float as the reference does."""
     def mm(a):
@@ -270,7 +324,7 @@ def convert_units(P, t, v, u, wepl):
     return t, v, u, mm(np.asarray(wepl, np.float32))
 
 
-def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=False):
+def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=False, tvs_sum=TVS_SUM_TILED_FP64):
     """The reference pipeline on the CPU.  Returns a dict with every intermediate result."""
     P = derive(P)
     out = {}
@@ -347,12 +401,19 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     x = x0.copy()
     x64 = x0.astype(np.float64)
     xs = []
-    for _ in range(its):
+    L, tvs_log = 0, []
+    for k in range(1, its + 1):  # DROP_GPU, DS.cu:9885-9915
+        if P.tvs_on and P.tvs_first:
+            x, L, tb, ta = tvs_iteration(P, hull, x, k, L, tvs_sum)
+            tvs_log.append((tb, ta, L))
         x = drop_iteration(P, off, vox, chord, w2, x, accum=accum)
+        if P.tvs_on and not P.tvs_first:
+            x, L, tb, ta = tvs_iteration(P, hull, x, k, L, tvs_sum)
+            tvs_log.append((tb, ta, L))
         if with_fp64:
             x64 = drop_iteration_fp64(P, off, vox, chord, w2, x64)
         xs.append(x.copy())
-    out.update(x=x, x_iterations=xs)
+    out.update(x=x, x_iterations=xs, tvs_log=tvs_log)
     if with_fp64:
         out["x_fp64"] = x64
     return out

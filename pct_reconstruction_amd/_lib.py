@@ -56,6 +56,9 @@ class Params(C.Structure):
         ("quirk_voxel_decode", _I), ("drop_accum", _I), ("median_filter_fbp", _I), ("fbp_med_filter_radius", _I),
         ("avg_filter_fbp", _I), ("avg_filter_hull", _I), ("hull_avg_filter_radius", _I), ("reserved0", _I),
         ("hull_avg_filter_threshold", _D),
+        ("tvs_on", _I), ("tvs_first", _I), ("tvs_conditioned", _I), ("tvs_old", _I), ("tvs_repetitions", _I),
+        ("tvs_l0", _I), ("tvs_seed", _I), ("reserved1", _I), ("tvs_a", _D), ("tvs_min_eta", _D),
+        ("tv_threshold", _D),
         # derived
         ("t_bins", _I), ("v_bins", _I), ("angular_bins", _I), ("num_bins", _I), ("slices", _I), ("num_voxels", _I),
         ("recon_cyl_height", _D), ("image_width", _D), ("image_height", _D), ("image_thickness", _D),
@@ -135,6 +138,8 @@ def lib() -> C.CDLL:
         "pct_get_histories": ([vp, C.c_int, vp, sz], C.c_int),
         "pct_get_lists": ([vp, vp, sz, vp, sz], C.c_int),
         "pct_residual": ([vp, P(dbl)], C.c_int),
+        "pct_get_tvs_log": ([vp, C.c_size_t, vp, vp, vp, P(i64)], C.c_int),
+        "pct_total_variation": ([vp, P(dbl)], C.c_int),
         "pct_metrics_json": ([vp], C.c_char_p),
         "pct_drop_iteration_device": ([i64, vp, vp, vp, vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_int, C.c_int,
                                        dbl, vp], C.c_int),

@@ -48,6 +48,15 @@ __global__ void k_to_float(int64_t n, const T* in, float* out)
     if (i < n) out[i] = (float)in[i];
 }
 
+// A new iterate restarts the DROP iteration count and the TVS exponent L (a global starting at 0 in
+// the reference, DS.h:439).
+void restart_iterations(pct_ctx* ctx)
+{
+    ctx->drop_k = 0;
+    ctx->tvs_L = 0;
+    ctx->tvs_log.clear();
+}
+
 struct StageTimer {
     pct_ctx* ctx;
     int slot;
@@ -155,6 +164,8 @@ int pct_ctx_destroy(pct_ctx* ctx)
     pct_free(&ctx->accum64); pct_free(&ctx->accum32); pct_free(&ctx->S); pct_free(&ctx->touched);
     pct_free(&ctx->n_touched); pct_free(&ctx->list_overflow);
     pct_drop_plan_free(&ctx->plan);
+    pct_free(&ctx->tvs_G); pct_free(&ctx->tvs_part); pct_free(&ctx->tvs_res);
+    if (ctx->tvs_res_h) (void)hipHostFree(ctx->tvs_res_h);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PCT_OK;
@@ -309,6 +320,7 @@ int pct_initial_iterate(pct_ctx* ctx, const float* x0_import)
     PCT_CHECK_HIP(ctx, hipGetLastError());
     tm.done();
     ctx->stage = ST_X0;
+    restart_iterations(ctx);
     return PCT_OK;
 }
 
@@ -405,6 +417,7 @@ int pct_set_image(pct_ctx* ctx, const float* x)
     if (!ctx->x) PCT_TRY(pct_alloc(ctx, &ctx->x, ctx->P.num_voxels));
     PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->x, x, ctx->P.num_voxels * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    restart_iterations(ctx);
     return PCT_OK;
 }
 
@@ -529,7 +542,12 @@ const char* pct_metrics_json(pct_ctx* ctx)
         std::snprintf(buf, sizeof(buf), "%s\"%s\": %.4f", k ? ", " : "", kTimerName[k], ctx->t_stage[k]);
         s += buf;
     }
-    std::snprintf(buf, sizeof(buf), "}, \"kernel_ms\": {\"mlp_lists\": %.4f}}", ctx->t_mlp_kernel);
+    std::snprintf(buf, sizeof(buf), "}, \"kernel_ms\": {\"mlp_lists\": %.4f}", ctx->t_mlp_kernel);
+    s += buf;
+    double tvs_ms = 0;
+    for (const TvsRecord& r : ctx->tvs_log) tvs_ms += r.ms;
+    std::snprintf(buf, sizeof(buf), ", \"tvs\": {\"iterations\": %zu, \"ms\": %.4f, \"L\": %d}}", ctx->tvs_log.size(),
+                  tvs_ms, ctx->tvs_L);
     s += buf;
     ctx->metrics = s;
     return ctx->metrics.c_str();
@@ -541,7 +559,28 @@ int pct_reset_iterate(pct_ctx* ctx)
     PCT_TRY(require(ctx, ST_X0, "pct_reset_iterate"));
     PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->x, ctx->x0, ctx->P.num_voxels * sizeof(float), hipMemcpyDeviceToDevice,
                                       ctx->stream));
+    restart_iterations(ctx);
     return PCT_OK;
+}
+
+int pct_get_tvs_log(pct_ctx* ctx, size_t capacity, double* tv_before, double* tv_after, int32_t* l_after,
+                    int64_t* count)
+{
+    if (!ctx || !count) return PCT_E_ARG;
+    *count = (int64_t)ctx->tvs_log.size();
+    for (size_t i = 0; i < ctx->tvs_log.size() && i < capacity; i++) {
+        if (tv_before) tv_before[i] = ctx->tvs_log[i].tv_before;
+        if (tv_after) tv_after[i] = ctx->tvs_log[i].tv_after;
+        if (l_after) l_after[i] = ctx->tvs_log[i].L;
+    }
+    return PCT_OK;
+}
+
+int pct_total_variation(pct_ctx* ctx, double* tv)
+{
+    if (!ctx || !tv) return PCT_E_ARG;
+    PCT_TRY(require(ctx, ST_X0, "pct_total_variation"));
+    return pct_stage_total_variation(ctx, tv);
 }
 
 int pct_drop_iteration_device(int64_t n, const int64_t* offsets, const uint32_t* voxels, const double* chord,

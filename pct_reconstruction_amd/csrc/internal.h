@@ -99,6 +99,12 @@ struct DropPlan {
     bool ready = false;
 };
 
+struct TvsRecord {
+    double tv_before, tv_after;  // TV of x before / after the NTVS iteration
+    int L;                       // exponent L after it
+    double ms;                   // device time of the iteration
+};
+
 struct pct_ctx {
     pct_params P;
     int device = 0;
@@ -180,6 +186,16 @@ struct pct_ctx {
     uint32_t* n_touched = nullptr;  // 2 counters (ping-pong)
     int32_t* list_overflow = nullptr;
 
+    // total variation superiorization between DROP iterations (k_tvs.hip)
+    float* tvs_G = nullptr;        // subgradient of the TV, NUM_VOXELS
+    double* tvs_part = nullptr;    // 2 x tiles of 1024 voxels: FP64 tile sums
+    double* tvs_res = nullptr;     // 2 reduced sums
+    double* tvs_res_h = nullptr;   // pinned host copy of tvs_res
+    int64_t tvs_nt = 0;
+    int tvs_L = 0;                 // the exponent L carried between iterations (DS.h:439)
+    int drop_k = 0;                // DROP iterations run since the initial iterate
+    std::vector<TvsRecord> tvs_log;
+
     // timing (ms) of the last run of each stage, and of the last MLP list kernel launch
     double t_stage[16] = {};
     double t_mlp_kernel = 0;
@@ -238,6 +254,8 @@ void pct_drop_plan_free(DropPlan* plan);
 int pct_drop_enter(pct_ctx* ctx, bool enter);  // image into (true) / out of the DROP layout
 int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing);
 int pct_stage_residual(pct_ctx* ctx, double* norm);
+int pct_tvs_iteration(pct_ctx* ctx, int k);               // NTVS_iteration for DROP iteration k (1-based)
+int pct_stage_total_variation(pct_ctx* ctx, double* tv);  // TV of ctx->x, fixed-order FP64 sum
 
 // compaction: keep entries with flag != 0 (flag array has ctx->n_hist entries); returns new count
 int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bool with_first_voxel);

@@ -214,9 +214,23 @@ int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* c
 int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing)
 {
     if (!ctx->plan.ready || ctx->plan.flags != ctx->drop_flags) PCT_TRY(pct_drop_prepare(ctx));
-    PCT_TRY(pct_drop_enter(ctx, true));
-    for (int it = 0; it < iterations; it++) PCT_TRY(pct_drop_iterate_plan(ctx, timing));
-    PCT_TRY(pct_drop_enter(ctx, false));
+    if (!ctx->P.tvs_on) {
+        PCT_TRY(pct_drop_enter(ctx, true));
+        for (int it = 0; it < iterations; it++) PCT_TRY(pct_drop_iterate_plan(ctx, timing));
+        PCT_TRY(pct_drop_enter(ctx, false));
+        ctx->drop_k += iterations;
+        return PCT_OK;
+    }
+    // DROP_GPU (DS.cu:9885-9915): NTVS_iteration before (TVS_FIRST) or after each DROP iteration; TVS
+    // works on the linear image, DROP on its own layout.
+    for (int it = 0; it < iterations; it++) {
+        const int k = ++ctx->drop_k;
+        if (ctx->P.tvs_first) PCT_TRY(pct_tvs_iteration(ctx, k));
+        PCT_TRY(pct_drop_enter(ctx, true));
+        PCT_TRY(pct_drop_iterate_plan(ctx, timing));
+        PCT_TRY(pct_drop_enter(ctx, false));
+        if (!ctx->P.tvs_first) PCT_TRY(pct_tvs_iteration(ctx, k));
+    }
     return PCT_OK;
 }
 

@@ -76,6 +76,18 @@ extern "C" void pct_params_default(pct_params* p)
     p->hull_avg_filter_radius = 0;
     p->reserved0 = 0;
     p->hull_avg_filter_threshold = 0.1;
+    // total variation superiorization: DS.h:427-444 except TVS_ON (off here, the plain method)
+    p->tvs_on = 0;
+    p->tvs_first = 1;
+    p->tvs_conditioned = 1;
+    p->tvs_old = 0;
+    p->tvs_repetitions = 5;
+    p->tvs_l0 = 0;
+    p->tvs_seed = 0;
+    p->reserved1 = 0;
+    p->tvs_a = 0.5;
+    p->tvs_min_eta = 1e-40;
+    p->tv_threshold = 0.0;
 }
 
 namespace {
@@ -114,6 +126,10 @@ const Key kKeys[] = {
     PK_B("AVG_FILTER_FBP", avg_filter_fbp), PK_B("AVG_FILTER_HULL", avg_filter_hull),
     PK_I("HULL_AVG_FILTER_RADIUS", hull_avg_filter_radius),
     PK_D("HULL_AVG_FILTER_THRESHOLD", hull_avg_filter_threshold),
+    PK_B("TVS_ON", tvs_on), PK_B("TVS_FIRST", tvs_first), PK_B("TVS_CONDITIONED", tvs_conditioned),
+    PK_B("TVS_OLD", tvs_old), PK_I("TVS_REPETITIONS", tvs_repetitions), PK_I("L_0", tvs_l0),
+    PK_I("TVS_SEED", tvs_seed), PK_D("A", tvs_a), PK_D("TVS_MIN_ETA", tvs_min_eta),
+    PK_D("TV_THRESHOLD", tv_threshold),
 };
 
 // Keys of settings.cfg that configure parts of the reference outside this path (I/O directories,
@@ -266,5 +282,9 @@ extern "C" int pct_params_derive(pct_params* p, char* err, size_t err_len)
     if (p->fbp_med_filter_radius < 0 || p->fbp_med_filter_radius > 10) return fail("FBP_MED_FILTER_RADIUS must be 0..10");
     if (p->hull_avg_filter_radius < 0 || p->hull_avg_filter_radius > 64)
         return fail("HULL_AVG_FILTER_RADIUS must be 0..64");
+    if (p->tvs_repetitions < 0 || p->tvs_repetitions > 10000) return fail("TVS_REPETITIONS must be 0..10000");
+    if (p->tvs_l0 < 0 || p->tvs_l0 > 300) return fail("L_0 must be 0..300");
+    if (!(p->tvs_a > 0 && p->tvs_a < 1)) return fail("A must lie in (0, 1)");
+    if (!(p->tvs_min_eta > 0 && p->tvs_min_eta < 1)) return fail("TVS_MIN_ETA must lie in (0, 1)");
     return PCT_OK;
 }

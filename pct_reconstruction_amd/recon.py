@@ -241,5 +241,22 @@ class Reconstruction:
         self._call("pct_residual", C.byref(r))
         return r.value
 
+    def total_variation(self) -> float:
+        """TV of the current iterate (calculate_total_variation, DS.cu:10165), fixed-order FP64 sum."""
+        r = C.c_double()
+        self._call("pct_total_variation", C.byref(r))
+        return r.value
+
+    def tvs_log(self) -> list:
+        """Per NTVS iteration since the initial iterate: (TV before, TV after, L after)."""
+        n = C.c_int64()
+        self._call("pct_get_tvs_log", 0, None, None, None, C.byref(n))
+        k = n.value
+        tb, ta = np.zeros(k, np.float64), np.zeros(k, np.float64)
+        L = np.zeros(k, np.int32)
+        self._call("pct_get_tvs_log", k, tb.ctypes.data_as(C.c_void_p), ta.ctypes.data_as(C.c_void_p),
+                   L.ctypes.data_as(C.c_void_p), C.byref(n))
+        return [(float(tb[i]), float(ta[i]), int(L[i])) for i in range(k)]
+
     def metrics(self) -> dict:
         return json.loads(lib().pct_metrics_json(self._ctx).decode())
