@@ -237,7 +237,8 @@ int pct_get_bins(pct_ctx* ctx, int kind, float* out);
 enum pct_hist_kind { PCT_H_X_ENTRY = 0, PCT_H_Y_ENTRY, PCT_H_Z_ENTRY, PCT_H_X_EXIT, PCT_H_Y_EXIT, PCT_H_Z_EXIT,
                      PCT_H_XY_ENTRY_ANGLE, PCT_H_XZ_ENTRY_ANGLE, PCT_H_XY_EXIT_ANGLE, PCT_H_XZ_EXIT_ANGLE,
                      PCT_H_WEPL, PCT_H_BIN_NUM /* int32 */, PCT_H_FIRST_VOXEL /* uint32 */,
-                     PCT_H_PATH_COUNT /* uint32 */, PCT_H_CHORD /* float */ };
+                     PCT_H_PATH_COUNT /* uint32 */, PCT_H_CHORD /* double */,
+                     PCT_H_GANTRY_ANGLE /* float, degrees: the angle loaded with the history */ };
 int pct_get_histories(pct_ctx* ctx, int kind, void* out, size_t capacity);
 /* MLP list of the reconstruction histories: offsets (n+1 int64) and voxel indices (uint32). */
 int pct_get_lists(pct_ctx* ctx, int64_t* offsets, size_t offsets_cap, uint32_t* voxels, size_t voxels_cap);
@@ -264,6 +265,39 @@ int pct_drop_iteration_device(int64_t n, const int64_t* offsets, const uint32_t*
                               double lambda, void* hip_stream);
 
 /* ---------------------------------------------------------------- I/O ---- */
+/*
+ * Preprocessed data of the reference's variant program (src/pCT_Reconstruction.cu, "V.cu"):
+ * accumulate_and_export_preprocessed_data (V.cu:6613-6658) with export_hull (:6540), export_x_0
+ * (:6551), export_histories (:6571), export_voxels_per_path (:6594) and export_avg_chord_lengths
+ * (:6603).  Writes into directory `dir` (names: include/Constants.h:84-88 and the write-mode
+ * extensions of include/Configurations.h:1802-1822):
+ *   MLP.bin               uint32 n, then every path's voxel indices, concatenated (uint32)
+ *   voxels_per_path.bin   uint32 n, then n uint32 path lengths
+ *   avg_chord_lengths.bin uint32 n, then n float effective chord lengths
+ *   WEPL.bin              uint32 n, then n float WEPL
+ *   histories.bin         uint32 n, then 15 arrays of n: entry voxel x, y, z (int32); x, y, z entry;
+ *                         x, y, z exit; xy, xz entry angle; xy, xz exit angle (float); bin number and
+ *                         gantry angle (int32)
+ *   hull.txt              NUM_VOXELS bytes, 1 = inside (binary despite the name, as the reference writes it)
+ *   x_0.txt               NUM_VOXELS floats (binary, likewise)
+ * for the n reconstruction histories in order (after pct_mlp_build_lists).  With
+ * PCT_EXPORT_CHORD_F64 it also writes avg_chord_lengths_f64.bin (uint32 n, n doubles): the chords at
+ * full precision, so that an import reproduces DROP bit for bit.
+ */
+enum pct_export_flags { PCT_EXPORT_CHORD_F64 = 1 };
+int pct_export_preprocessed(pct_ctx* ctx, const char* dir, int flags);
+/*
+ * init_import_preprocessed_data (V.cu:6775-6861), import_hull (:6659) and import_x_0 (:6672): reads
+ * the files above into a context at any stage and leaves it ready for pct_drop_iterate, with x = x0.
+ * MLP.bin, voxels_per_path.bin, avg_chord_lengths.bin (or _f64), WEPL.bin and x_0.txt are
+ * required; the four per-history files must agree on n (V.cu:6810).  hull.txt (needed by TVS) and
+ * histories.bin are optional.  MLP.bin and WEPL.bin may carry the "%u
+" text header that
+ * accumulate_and_export_preprocessed_data writes (V.cu:6628-6629) in place of the uint32.  Voxel
+ * indices >= NUM_VOXELS and paths longer than 4 (COLUMNS + ROWS + SLICES) + 16 are rejected.
+ */
+int pct_import_preprocessed(pct_ctx* ctx, const char* dir);
+
 /* Read one PCTD/PCTN projection file (count_histories_v0 :3578, read_data_chunk_v0 :4361,
  * format SupplementalDocumentation/ProposedDataFormat_Version1-1.txt).  With arrays NULL only
  * the header is read into *n and *angle_deg.  Version 1 / PCTN files carry per-history angles

@@ -267,6 +267,65 @@ def residual(off, vox, chord, wepl, x, min_len):
 TVS_SUM_SERIAL_FLOAT, TVS_SUM_TILED_FP64 = 0, 1
 
 
+def write_preprocessed(P, out, directory, text_header=False, f64=True):
+    """The preprocessed-data files of accumulate_and_export_preprocessed_data (V.cu:6613-6658) and the
+    exports it calls (V.cu:6540-6612), from run_pipeline's results.  Every file starts with the
+    history count as uint32; text_header=True writes MLP.bin and WEPL.bin with the exporter's "%u\n"
+    header instead (V.cu:6628-6629)."""
+    import struct
+    from pathlib import Path
+    d = Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    n = int(out["chord"].shape[0])
+    cnt = np.diff(out["offsets"]).astype("<u4")
+    u32 = struct.pack("<I", n)
+    hdr = f"{n}\n".encode() if text_header else u32
+    (d / "MLP.bin").write_bytes(hdr + np.asarray(out["voxels"], "<u4").tobytes())
+    (d / "voxels_per_path.bin").write_bytes(u32 + cnt.tobytes())
+    (d / "avg_chord_lengths.bin").write_bytes(u32 + np.asarray(out["chord"], "<f8").astype("<f4").tobytes())
+    if f64:
+        (d / "avg_chord_lengths_f64.bin").write_bytes(u32 + np.asarray(out["chord"], "<f8").tobytes())
+    (d / "WEPL.bin").write_bytes(hdr + np.asarray(out["wepl_recon"], "<f4").tobytes())
+    fv = np.asarray(out["first_voxel"], np.int64)
+    C, R = P.columns, P.rows
+    parts = [u32] + [np.asarray(a, "<i4").tobytes() for a in (fv % C, (fv // C) % R, fv // (C * R))]
+    parts += [np.asarray(out["h_recon"][k], "<f4").tobytes() for k in range(10)]
+    parts += [np.asarray(out["bin_recon"], "<i4").tobytes(),
+              np.asarray(out["gantry_recon"], np.float32).astype("<i4").tobytes()]
+    (d / "histories.bin").write_bytes(b"".join(parts))
+    (d / "hull.txt").write_bytes(np.asarray(out["hull"], np.uint8).tobytes())
+    (d / "x_0.txt").write_bytes(np.asarray(out["x0"], "<f4").tobytes())
+
+
+def read_preprocessed(directory):
+    """Reads the files of write_preprocessed (either header form).  Returns a dict of arrays."""
+    import struct
+    from pathlib import Path
+    d = Path(directory)
+    raw = (d / "voxels_per_path.bin").read_bytes()
+    n = struct.unpack_from("<I", raw)[0]
+    cnt = np.frombuffer(raw, "<u4", count=n, offset=4)
+
+    def payload(name, nbytes):
+        b = (d / name).read_bytes()
+        if len(b) == 4 + nbytes and struct.unpack_from("<I", b)[0] == n:
+            return b[4:]
+        k = b.index(b"\n")
+        assert int(b[:k]) == n and len(b) == k + 1 + nbytes, name
+        return b[k + 1:]
+
+    total = int(cnt.sum(dtype=np.int64))
+    res = dict(n=n, counts=cnt, voxels=np.frombuffer(payload("MLP.bin", 4 * total), "<u4"),
+               wepl=np.frombuffer(payload("WEPL.bin", 4 * n), "<f4"),
+               chord32=np.frombuffer((d / "avg_chord_lengths.bin").read_bytes(), "<f4", count=n, offset=4))
+    hb = (d / "histories.bin").read_bytes()
+    assert len(hb) == 4 + 15 * 4 * n
+    arr = [np.frombuffer(hb, "<i4" if k in (0, 1, 2, 13, 14) else "<f4", count=n, offset=4 + 4 * n * k)
+           for k in range(15)]
+    res.update(voxel_xyz=arr[:3], h=arr[3:13], bin=arr[13], gantry=arr[14])
+    return res
+
+
 def tile_sum(terms):
     """FP64 sum in the library's fixed order (tiles of 1024, halving trees)."""
     t = np.ascontiguousarray(terms, np.float64)
@@ -332,6 +391,7 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     h, missed = intersections(P, t, v, u, data["angle_deg"])
     bin_num, in_range = binning(P, h)
     keep = ~missed
+    ids = np.nonzero(keep)[0]  # loaded-history index of each surviving history
     h = np.ascontiguousarray(h[:, keep])
     w = np.ascontiguousarray(w[keep])
     bin_num = np.ascontiguousarray(bin_num[keep])
@@ -350,6 +410,8 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     st = stats_and_cuts(P, bin_num, in_range, w, h)
     out.update(st)
     keep = ~st["failed"]
+    ids = ids[keep]
+    bin_cut = bin_num[keep]
     h = np.ascontiguousarray(h[:, keep])
     w = np.ascontiguousarray(w[keep])
     out.update(h_cut=h, wepl_cut=w)
@@ -386,6 +448,9 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
         return out
     h2, first = endpoints(P, hull_ep, h)
     keep = first != P.num_voxels
+    ids = ids[keep]
+    out.update(bin_recon=np.ascontiguousarray(bin_cut[keep]),
+               gantry_recon=np.ascontiguousarray(np.asarray(data["angle_deg"], np.float32)[ids]))
     h2 = np.ascontiguousarray(h2[:, keep])
     w2 = np.ascontiguousarray(w[keep])
     first = np.ascontiguousarray(first[keep])

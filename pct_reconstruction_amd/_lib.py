@@ -30,7 +30,7 @@ BINS = {"counts_pre": 0, "mean_wepl": 1, "mean_ut": 2, "mean_uv": 3, "std_wepl":
         "counts_post": 7, "sinogram": 8, "sinogram_filtered": 9}
 HIST = {"x_entry": 0, "y_entry": 1, "z_entry": 2, "x_exit": 3, "y_exit": 4, "z_exit": 5, "xy_entry_angle": 6,
         "xz_entry_angle": 7, "xy_exit_angle": 8, "xz_exit_angle": 9, "wepl": 10, "bin_num": 11, "first_voxel": 12,
-        "path_count": 13, "chord": 14}
+        "path_count": 13, "chord": 14, "gantry_angle": 15}
 HIST_DTYPE = {"bin_num": np.int32, "first_voxel": np.uint32, "path_count": np.uint32, "chord": np.float64}
 COUNTS = {"loaded": 0, "recon_volume": 1, "post_cuts": 2, "reconstruction": 3, "list_entries": 4, "current": 5}
 
@@ -140,6 +140,8 @@ def lib() -> C.CDLL:
         "pct_residual": ([vp, P(dbl)], C.c_int),
         "pct_get_tvs_log": ([vp, C.c_size_t, vp, vp, vp, P(i64)], C.c_int),
         "pct_total_variation": ([vp, P(dbl)], C.c_int),
+        "pct_export_preprocessed": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "pct_import_preprocessed": ([vp, C.c_char_p], C.c_int),
         "pct_metrics_json": ([vp], C.c_char_p),
         "pct_drop_iteration_device": ([i64, vp, vp, vp, vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_int, C.c_int,
                                        dbl, vp], C.c_int),

@@ -144,6 +144,7 @@ int pct_ctx_destroy(pct_ctx* ctx)
         pct_free(&ctx->h_tmp[k]);
     }
     pct_free(&ctx->wepl); pct_free(&ctx->wepl_tmp); pct_free(&ctx->bin_num); pct_free(&ctx->bin_tmp);
+    pct_free(&ctx->gantry); pct_free(&ctx->gantry_tmp);
     pct_free(&ctx->in_range); pct_free(&ctx->u8_tmp); pct_free(&ctx->flag); pct_free(&ctx->pos);
     pct_free(&ctx->first_voxel); pct_free(&ctx->u32_a); pct_free(&ctx->u32_b); pct_free(&ctx->u32_c);
     pct_free(&ctx->u32_d); pct_free(&ctx->cub_tmp);
@@ -468,6 +469,7 @@ int pct_get_histories(pct_ctx* ctx, int kind, void* out, size_t capacity)
     size_t esz = 4;
     if (kind >= PCT_H_X_ENTRY && kind <= PCT_H_XZ_EXIT_ANGLE) src = ctx->h[kind];
     else if (kind == PCT_H_WEPL) src = ctx->wepl;
+    else if (kind == PCT_H_GANTRY_ANGLE) src = ctx->gantry;
     else if (kind == PCT_H_BIN_NUM) src = ctx->bin_num;
     else if (kind == PCT_H_FIRST_VOXEL) src = ctx->first_voxel;
     else if (kind == PCT_H_PATH_COUNT) src = ctx->stage >= ST_LISTS ? ctx->list_cnt : nullptr;

@@ -122,12 +122,14 @@ struct pct_ctx {
     int64_t n_recon_volume = 0, n_post_cuts = 0, n_recon = 0;
     float* h[10] = {};        // x/y/z entry, x/y/z exit, xy/xz entry angle, xy/xz exit angle
     float* wepl = nullptr;
+    float* gantry = nullptr;  // gantry angle of the history's projection [deg]
     int32_t* bin_num = nullptr;
     uint8_t* in_range = nullptr;
     uint32_t* first_voxel = nullptr;
     // scratch of the same capacity
     float* h_tmp[10] = {};
     float* wepl_tmp = nullptr;
+    float* gantry_tmp = nullptr;
     int32_t* bin_tmp = nullptr;
     uint8_t* u8_tmp = nullptr;
     uint8_t* flag = nullptr;
@@ -259,6 +261,8 @@ int pct_stage_total_variation(pct_ctx* ctx, double* tv);  // TV of ctx->x, fixed
 
 // compaction: keep entries with flag != 0 (flag array has ctx->n_hist entries); returns new count
 int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bool with_first_voxel);
+// per-history arrays (and their compaction scratch) for n histories
+int pct_ensure_hist_capacity(pct_ctx* ctx, int64_t n);
 // per-kernel timing of DROP launches (HIP events around every launch on the launch stream)
 struct DropTiming {
     double scatter_ms = 0, apply_ms = 0;

@@ -329,6 +329,8 @@ int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bo
     }
     hipLaunchKernelGGL(k_scatter<float>, dim3(g), dim3(kBlock), 0, s, n, keep, ctx->pos, ctx->wepl, ctx->wepl_tmp);
     std::swap(ctx->wepl, ctx->wepl_tmp);
+    hipLaunchKernelGGL(k_scatter<float>, dim3(g), dim3(kBlock), 0, s, n, keep, ctx->pos, ctx->gantry, ctx->gantry_tmp);
+    std::swap(ctx->gantry, ctx->gantry_tmp);
     hipLaunchKernelGGL(k_scatter<int32_t>, dim3(g), dim3(kBlock), 0, s, n, keep, ctx->pos, ctx->bin_num, ctx->bin_tmp);
     std::swap(ctx->bin_num, ctx->bin_tmp);
     hipLaunchKernelGGL(k_scatter<uint8_t>, dim3(g), dim3(kBlock), 0, s, n, keep, ctx->pos, ctx->in_range, ctx->u8_tmp);
@@ -344,7 +346,7 @@ int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bo
     return PCT_OK;
 }
 
-static int ensure_hist_capacity(pct_ctx* ctx, int64_t n)
+int pct_ensure_hist_capacity(pct_ctx* ctx, int64_t n)
 {
     if (n <= ctx->cap_hist) return PCT_OK;
     for (int k = 0; k < 10; k++) {
@@ -353,6 +355,8 @@ static int ensure_hist_capacity(pct_ctx* ctx, int64_t n)
     }
     PCT_TRY(pct_alloc(ctx, &ctx->wepl, n));
     PCT_TRY(pct_alloc(ctx, &ctx->wepl_tmp, n));
+    PCT_TRY(pct_alloc(ctx, &ctx->gantry, n));
+    PCT_TRY(pct_alloc(ctx, &ctx->gantry_tmp, n));
     PCT_TRY(pct_alloc(ctx, &ctx->bin_num, n));
     PCT_TRY(pct_alloc(ctx, &ctx->bin_tmp, n));
     PCT_TRY(pct_alloc(ctx, &ctx->in_range, n));
@@ -375,7 +379,7 @@ int pct_stage_intersect_bin_carve(pct_ctx* ctx)
     const pct_params& P = ctx->P;
     hipStream_t s = ctx->stream;
     const int64_t n = ctx->n_loaded;
-    PCT_TRY(ensure_hist_capacity(ctx, n));
+    PCT_TRY(pct_ensure_hist_capacity(ctx, n));
     unsigned g = grid_for(n);
     RawPtrs r{ctx->raw[0], ctx->raw[1], ctx->raw[2], ctx->raw[3], ctx->raw[4], ctx->raw[5], ctx->raw[6],
               ctx->raw[7], ctx->raw[8], ctx->raw[9], ctx->raw[10], ctx->raw[11], ctx->raw[13]};
@@ -387,6 +391,7 @@ int pct_stage_intersect_bin_carve(pct_ctx* ctx)
                            P.recon_cyl_height, P.pi / 180.0);
         // WEPL travels with the histories
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->wepl, ctx->raw[12], n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(ctx->gantry, ctx->raw[13], n * sizeof(float), hipMemcpyDeviceToDevice, s));
         // angle-bin centres: angle_bin * ANGULAR_BIN_SIZE * ANGLE_TO_RADIANS (DS.cu:5506)
         std::vector<double> cs(2 * P.angular_bins);
         for (int a = 0; a < P.angular_bins; a++) {

@@ -241,6 +241,15 @@ class Reconstruction:
         self._call("pct_residual", C.byref(r))
         return r.value
 
+    def export_preprocessed(self, directory, exact_chords: bool = True) -> None:
+        """accumulate_and_export_preprocessed_data (V.cu:6613): path lists, per-history data, hull, x0."""
+        Path(directory).mkdir(parents=True, exist_ok=True)
+        self._call("pct_export_preprocessed", str(directory).encode(), 1 if exact_chords else 0)
+
+    def import_preprocessed(self, directory) -> None:
+        """init_import_preprocessed_data (V.cu:6775): ready for DROP from the exported files."""
+        self._call("pct_import_preprocessed", str(directory).encode())
+
     def total_variation(self) -> float:
         """TV of the current iterate (calculate_total_variation, DS.cu:10165), fixed-order FP64 sum."""
         r = C.c_double()
