@@ -368,6 +368,10 @@ int pct_net_load(pct_net* net, const char* path);
 /* Replace the initial iterate of `ctx` (after pct_mlp_build_lists) by the net's x0 when its DROP data
  * residual ||Ax - b|| is below that of the current x0; x is reset to the chosen x0. */
 int pct_net_apply(pct_net* net, pct_ctx* ctx, int* used_net, double* residual_fbp, double* residual_net);
+/* The same judgement for a net whose input is the current iterate x (e.g. after a few DROP iterations
+ * from the FBP start, for a net trained on such iterates): x is replaced by net(x, hull) when that
+ * lowers ||Ax - b||; x0 is left unchanged. */
+int pct_net_apply_iterate(pct_net* net, pct_ctx* ctx, int* used_net, double* residual_x, double* residual_net);
 
 #ifdef __cplusplus
 }

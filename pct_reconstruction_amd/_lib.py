@@ -141,6 +141,7 @@ def lib() -> C.CDLL:
         "pct_get_tvs_log": ([vp, C.c_size_t, vp, vp, vp, P(i64)], C.c_int),
         "pct_total_variation": ([vp, P(dbl)], C.c_int),
         "pct_export_preprocessed": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "pct_net_apply_iterate": ([vp, vp, P(C.c_int), P(dbl), P(dbl)], C.c_int),
         "pct_import_preprocessed": ([vp, C.c_char_p], C.c_int),
         "pct_metrics_json": ([vp], C.c_char_p),
         "pct_drop_iteration_device": ([i64, vp, vp, vp, vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -1034,4 +1034,41 @@ int pct_net_apply(pct_net* net, pct_ctx* ctx, int* used_net, double* residual_fb
     return PCT_OK;
 }
 
+int pct_net_apply_iterate(pct_net* net, pct_ctx* ctx, int* used_net, double* residual_x, double* residual_net)
+{
+    if (!net || !ctx) return PCT_E_ARG;
+    const pct_params& P = ctx->P;
+    if (P.columns != net->W || P.rows != net->H || P.slices != net->D) {
+        net->err = "network grid differs from the reconstruction grid";
+        return PCT_E_ARG;
+    }
+    if (ctx->stage < ST_LISTS || !ctx->hull || !ctx->x) {
+        net->err = "pct_net_apply_iterate needs the hull, an iterate and the MLP lists";
+        return PCT_E_STATE;
+    }
+    const int64_t n0 = net->nv[0];
+    NET_HIP(net, hipStreamSynchronize(ctx->stream));
+    PCT_TRY(net_forward(net, ctx->x, ctx->hull, nullptr, nullptr));
+    NET_HIP(net, hipStreamSynchronize(net->stream));
+    hipStream_t cs = ctx->stream;
+    double r_x = 0, r_net = 0;
+    if (pct_stage_residual(ctx, &r_x) != PCT_OK) {
+        net->err = ctx->err;
+        return PCT_E_HIP;
+    }
+    NET_HIP(net, hipMemcpyAsync(net->fbp, ctx->x, n0 * sizeof(float), hipMemcpyDeviceToDevice, cs));  // keep x
+    NET_HIP(net, hipMemcpyAsync(ctx->x, net->x0, n0 * sizeof(float), hipMemcpyDeviceToDevice, cs));
+    if (pct_stage_residual(ctx, &r_net) != PCT_OK) {
+        net->err = ctx->err;
+        return PCT_E_HIP;
+    }
+    const int use = r_net < r_x;
+    if (!use) NET_HIP(net, hipMemcpyAsync(ctx->x, net->fbp, n0 * sizeof(float), hipMemcpyDeviceToDevice, cs));
+    NET_HIP(net, hipStreamSynchronize(cs));
+    if (used_net) *used_net = use;
+    if (residual_x) *residual_x = r_x;
+    if (residual_net) *residual_net = r_net;
+    return PCT_OK;
+}
+
 }  // extern "C"

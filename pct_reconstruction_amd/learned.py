@@ -101,6 +101,13 @@ class WarmStartNet:
     def load(self, path: str | Path) -> None:
         self._check(lib().pct_net_load(self._net, str(path).encode()), "load")
 
+    def apply_iterate(self, recon) -> dict:
+        """Replace the current iterate of `recon` by net(x, hull) when that lowers ||Ax - b||."""
+        used, rx, rn = C.c_int(), C.c_double(), C.c_double()
+        self._check(lib().pct_net_apply_iterate(self._net, recon._ctx, C.byref(used), C.byref(rx), C.byref(rn)),
+                    "apply_iterate")
+        return {"used_net": bool(used.value), "residual_x": rx.value, "residual_net": rn.value}
+
     def apply(self, recon) -> dict:
         """Install the net's x0 in `recon` (after build_mlp_lists) when it lowers ||Ax - b||."""
         used, rf, rn = C.c_int(), C.c_double(), C.c_double()

@@ -11,6 +11,11 @@ after 40 iterations from the FBP start).  Evaluation per held-out seed:
                 residual is <= r_ref (at least one) -> iterations k, time t_net (net inference and the
                 residual judgement included).
 Reported: k, t_fbp / t_net, the relative L2 difference between the two final images.
+
+With --input-iterations m > 0 the net's input is the iterate after m DROP iterations from the FBP
+start instead of the FBP image (training pair = (x_m, hull, DROP image after 40 iterations)); the
+learned start then runs m iterations, applies the net to x_m (kept only if it lowers ||Ax - b||) and
+continues; its iteration count and time include the m iterations.
 """
 from __future__ import annotations
 
@@ -39,20 +44,35 @@ def _prepare(seed: int, n: int):
     return P, r
 
 
-def training_pair(seed: int, n: int, iterations: int = 40):
-    """(FBP, hull, DROP image after `iterations` from the FBP start) of phantom `seed`."""
+def training_pair(seed: int, n: int, iterations: int = 40, input_iterations: int = 0):
+    """(net input, hull, DROP image after `iterations` from the FBP start) of phantom `seed`; the input
+    is the FBP image, or the iterate after `input_iterations` DROP iterations."""
     P, r = _prepare(seed, n)
     try:
-        fbp = r.image("fbp")
         hull = r.image("hull").astype(np.uint8)
-        r.DROP(iterations)
+        if input_iterations > 0:
+            r.DROP(input_iterations)
+            inp = r.image("x")
+            r.DROP(iterations - input_iterations)
+        else:
+            inp = r.image("fbp")
+            r.DROP(iterations)
         target = r.image("x")
     finally:
         r.close()
-    return fbp, hull, target
+    return inp, hull, target
 
 
-def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int, long_iterations: int = 40) -> dict:
+def _learned_start(net: WarmStartNet, r, m: int) -> dict:
+    if m > 0:
+        r.DROP(m)
+        j = net.apply_iterate(r)
+        return {"used_net": j["used_net"], "residual_net": j["residual_net"], "residual_fbp": j["residual_x"]}
+    return net.apply(r)
+
+
+def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int, long_iterations: int = 40,
+             input_iterations: int = 0) -> dict:
     P, r = _prepare(seed, n)
     try:
         hull = r.image("hull").astype(bool)
@@ -72,10 +92,10 @@ def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int, long_ite
         r.reset_iterate()
         r0_fbp = r.residual()
         # learned start: count the iterations needed (residual checked after each)
-        judge = net.apply(r)
-        k = 0
+        judge = _learned_start(net, r, input_iterations)
+        k = input_iterations
         res = judge["residual_net"] if judge["used_net"] else judge["residual_fbp"]
-        while k < 1 or (res > r_ref and k < 4 * ref_iterations):
+        while k < input_iterations + 1 or (res > r_ref and k < 4 * ref_iterations):
             r.DROP(1)
             k += 1
             res = r.residual()
@@ -84,8 +104,8 @@ def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int, long_ite
         r.reset_iterate()
         r.synchronize()
         t0 = time.perf_counter()
-        net.apply(r)
-        r.DROP(k)
+        _learned_start(net, r, input_iterations)
+        r.DROP(k - input_iterations)
         r.synchronize()
         t_net = time.perf_counter() - t0
     finally:
@@ -114,6 +134,8 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--weights", default=None, help="write (or with --eval-only read) the weights here")
     ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--input-iterations", type=int, default=1,
+                    help="net input: FBP (0) or the iterate after this many DROP iterations")
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
 
@@ -122,13 +144,13 @@ def main(argv=None) -> int:
     report = {"workload": f"learned-128: random ellipsoid phantoms, {P.columns}x{P.rows}x{P.slices} @ 2 mm, "
                           f"{args.histories} histories each, MSC hull, FBP x0",
               "train": args.train, "test": args.test, "target_iterations": args.target_iterations,
-              "ref_iterations": args.ref_iterations}
+              "ref_iterations": args.ref_iterations, "input_iterations": args.input_iterations}
     if args.eval_only:
         net.load(args.weights)
     else:
         t0 = time.perf_counter()
         for s in range(args.train):
-            net.add_sample(*training_pair(s, args.histories, args.target_iterations))
+            net.add_sample(*training_pair(s, args.histories, args.target_iterations, args.input_iterations))
             if (s + 1) % 25 == 0:
                 print(f"[warmstart] {s + 1}/{args.train} training pairs ({time.perf_counter() - t0:.0f} s)",
                       flush=True)
@@ -145,7 +167,7 @@ def main(argv=None) -> int:
             net.save(args.weights)
     evals = []
     for j in range(args.test):
-        e = evaluate(net, 100000 + j, args.histories, args.ref_iterations)
+        e = evaluate(net, 100000 + j, args.histories, args.ref_iterations, input_iterations=args.input_iterations)
         evals.append(e)
         print(f"[warmstart] test {j}: {json.dumps(e)}", flush=True)
     sp = [e["speedup"] for e in evals if e["speedup"]]

@@ -128,3 +128,37 @@ def test_net_training_reduces_loss_and_weights_round_trip(oracle, tmp_path):
         with WarmStartNet(shape[2], shape[1], shape[0], seed=9) as net2:
             net2.load(path)
             np.testing.assert_array_equal(net2.predict(fbp, hull), x_a)
+
+
+@pytest.mark.gpu
+def test_committed_weights_reach_the_fbp_start_residual_in_two_iterations():
+    """The shipped net (learned-128, input = the iterate after one DROP iteration) on a held-out
+    phantom: the net's image lowers ||Ax - b|| below the FBP start's 20-iteration residual, so the
+    learned start needs 1 + 1 iterations instead of 20."""
+    from pathlib import Path
+
+    from pct_reconstruction_amd import configs, synth
+    from pct_reconstruction_amd.learned import WarmStartNet
+    from pct_reconstruction_amd.recon import Reconstruction
+
+    weights = Path(__file__).resolve().parent.parent / "pct_reconstruction_amd" / "weights" / \
+        "warmstart_learned128_x1.bin"
+    P, ph, bm, n = configs.learned_128(100019, 10_000_000)
+    d = synth.generate(ph, bm, 100019, n)
+    with Reconstruction(P) as r, WarmStartNet(P.columns, P.rows, P.slices, seed=0) as net:
+        net.load(str(weights))
+        r.load_histories(d["t"], d["v"], d["u"], d["wepl"], d["angle_deg"])
+        r.preprocessing()
+        r.reconstruction_cuts()
+        r.build_mlp_lists()
+        r.DROP(20)
+        r_ref = r.residual()
+        r.reset_iterate()
+        r.DROP(1)
+        j = net.apply_iterate(r)
+        r.DROP(1)
+        res = r.residual()
+    print(f"[warm start] r_ref {r_ref:.2f}; x1 {j['residual_x']:.2f} -> net {j['residual_net']:.2f}; "
+          f"after one more iteration {res:.2f}", flush=True)
+    assert j["used_net"] and j["residual_net"] < j["residual_x"]
+    assert res <= r_ref
