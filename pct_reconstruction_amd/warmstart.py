@@ -151,9 +151,7 @@ def main(argv=None) -> int:
         t0 = time.perf_counter()
         for s in range(args.train):
             net.add_sample(*training_pair(s, args.histories, args.target_iterations, args.input_iterations))
-            if (s + 1) % 25 == 0:
-                print(f"[warmstart] {s + 1}/{args.train} training pairs ({time.perf_counter() - t0:.0f} s)",
-                      flush=True)
+            print(f"[warmstart] {s + 1}/{args.train} training pairs ({time.perf_counter() - t0:.0f} s)", flush=True)
         report["data_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         losses = []
