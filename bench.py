@@ -9,6 +9,9 @@ plus 20 DROP iterations from x0, all inputs resident in HBM.
 metric: MLP+DROP proton histories/s = reconstruction histories x DROP iterations / step time.
 The JSON line also carries the roofline of the dominant kernel (HIP-event timing of every launch of
 that kernel during one profiled iteration) and the oracle's CPU rate on a bounded sample.
+
+--workload large-200M-segment measures one GPU's share of the large-200M run instead: a 25M-history
+segment on the 512x512x128 grid (not the default line).
 """
 from __future__ import annotations
 
@@ -65,7 +68,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--histories", type=int, default=50_000_000)
+    ap.add_argument("--workload", default="head-50M", choices=["head-50M", "large-200M-segment"],
+                    help="head-50M (the benchmark) or one 25M-history segment of the large-200M run "
+                         "(512x512x128 @ 0.5x0.5x1 mm, 230 MeV, SM hull, block 120000, 12 iterations)")
+    ap.add_argument("--histories", type=int, default=None, help="default: 50M (head-50M), 25M (segment)")
     ap.add_argument("--iterations", type=int, default=None, help="DROP iterations per step (default: config)")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="reconstruction histories timed on the CPU (0: all of them)")
@@ -80,7 +86,13 @@ def main() -> int:
 
     from pct_reconstruction_amd import Reconstruction, configs, synth
 
-    P, ph, bm, n = configs.head_50m(args.histories)
+    if args.workload == "head-50M":
+        P, ph, bm, n = configs.head_50m(args.histories or 50_000_000)
+        workload = "head-50M: 256x256x96 @ ~1 mm, MSC+SM hulls, 3-sigma cuts, FBP x0"
+    else:
+        P, ph, bm, n = configs.large_200m(args.histories or 25_000_000)
+        workload = ("large-200M segment: 25M of the 200M histories, 512x512x128 @ 0.5x0.5x1 mm, 230 MeV, SM hull, "
+                    "3-sigma cuts, FBP x0")
     iters = P.iterations if args.iterations is None else args.iterations
     log(f"generating {n} histories (seed {args.seed})")
     t0 = time.perf_counter()
@@ -186,8 +198,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64 (MLP, accumulators) / f32 (image)",
         "data": "synthetic (seeded 10-ellipsoid head phantom, Highland scatter, WEPL noise 2.5 mm, 1% outliers)",
-        "config": {"workload": "head-50M: 256x256x96 @ ~1 mm, MSC+SM hulls, 3-sigma cuts, FBP x0, "
-                               f"MLP + DROP {iters} iterations, block {P.drop_block_size}",
+        "config": {"workload": f"{workload}, MLP + DROP {iters} iterations, block {P.drop_block_size}",
                    "histories_loaded": n, "histories_reconstructed": n_recon, "list_entries": entries,
                    "drop_iterations": iters, "drop_block_size": P.drop_block_size},
         "roofline": roofline,

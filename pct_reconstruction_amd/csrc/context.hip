@@ -539,6 +539,11 @@ const char* pct_metrics_json(pct_ctx* ctx)
                   (long long)ctx->n_loaded, (long long)ctx->n_recon_volume, (long long)ctx->n_post_cuts,
                   (long long)ctx->n_recon);
     s += buf;
+    size_t mem_free = 0, mem_total = 0;
+    (void)hipMemGetInfo(&mem_free, &mem_total);
+    std::snprintf(buf, sizeof(buf), ", \"list_entries\": %lld, \"list_capacity\": %lld, \"device_bytes_used\": %lld",
+                  (long long)ctx->list_entries, (long long)ctx->list_cap, (long long)(mem_total - mem_free));
+    s += buf;
     s += "}, \"stage_ms\": {";
     for (int k = 0; k <= TM_DROP; k++) {
         std::snprintf(buf, sizeof(buf), "%s\"%s\": %.4f", k ? ", " : "", kTimerName[k], ctx->t_stage[k]);

@@ -44,14 +44,16 @@ __device__ __forceinline__ uint32_t slab_index(uint32_t per)
     return per ? (blockIdx.x % kXcds) * per + blockIdx.x / kXcds : blockIdx.x;
 }
 
-// list entries (all `n` of the CSR capacity; unused slots hold no voxel and are mapped harmlessly)
+// list entries (all `n` of the CSR capacity; unused slots hold no voxel and are mapped harmlessly).
+// Grid-stride: the capacity can exceed 2^32 entries, more work-items than one dispatch can hold.
 __global__ void k_bricks_list(int64_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, BrickGeom g,
                               uint32_t nv)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = in[i];
-    out[i] = v < nv ? brick_of(v, g) : 0u;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t v = in[i];
+        out[i] = v < nv ? brick_of(v, g) : 0u;
+    }
 }
 
 // image to (to_bricks = 1) or from brick order
@@ -168,14 +170,15 @@ __global__ void k_group_emit(int64_t h0, int64_t n, int64_t BS, int64_t rb0, int
 
 // runs of the sorted keys: voxel of each run, and the first run of every block of the group
 __global__ void k_group_runs(const uint32_t* __restrict__ run_keys, const uint32_t* __restrict__ n_runs, int vbits,
-                             uint32_t* __restrict__ t_vox, int32_t* __restrict__ run_start)
+                             uint32_t n_blocks, uint32_t* __restrict__ t_vox, int32_t* __restrict__ run_start)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= *n_runs) return;
     const uint32_t key = run_keys[u];
     const uint32_t lb = key >> vbits;
     t_vox[u] = key & ((1u << vbits) - 1u);
-    if (u == 0 || (run_keys[u - 1] >> vbits) != lb) run_start[lb] = (int32_t)u;
+    // lb < n_blocks for every key k_group_emit writes; the host checks the runs it gets
+    if (lb < n_blocks && (u == 0 || (run_keys[u - 1] >> vbits) != lb)) run_start[lb] = (int32_t)u;
 }
 
 // per history: u_i (DS.cu:9321-9380 with the list of find_MLP_path_GPU_tabulated).
@@ -559,7 +562,8 @@ int pct_drop_prepare(pct_ctx* ctx)
         }
         const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
         if (!D.vox_b_ready && ctx->list_cap > 0)  // lists built without the brick index
-            hipLaunchKernelGGL(k_bricks_list, dim3(grid_for(ctx->list_cap)), dim3(kBlock), 0, s, ctx->list_cap,
+            hipLaunchKernelGGL(k_bricks_list, dim3(std::min(grid_for(ctx->list_cap), 1u << 20)), dim3(kBlock), 0, s,
+                               ctx->list_cap,
                                ctx->list_vox, D.vox_b, bg, (uint32_t)P.num_voxels);
         D.vox_b_ready = true;
         lvox = D.vox_b;
@@ -625,7 +629,7 @@ int pct_drop_prepare(pct_ctx* ctx)
             PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.runs, D.t_off + run_off + gi,
                                                                 (int)(E + 1), s));
             hipLaunchKernelGGL(k_group_runs, dim3(grid_for(E)), dim3(kBlock), 0, s, D.keys, D.n_runs, vbits,
-                               D.t_vox + run_off + gi, D.run_start);
+                               (uint32_t)g.n_blocks, D.t_vox + run_off + gi, D.run_start);
             PCT_CHECK_HIP(ctx, hipGetLastError());
             PCT_CHECK_HIP(ctx, hipMemcpyAsync(&n_runs_h[gi], D.n_runs, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
@@ -633,6 +637,11 @@ int pct_drop_prepare(pct_ctx* ctx)
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(run_start_h[gi].data(), D.run_start, g.n_blocks * sizeof(int32_t),
                                           hipMemcpyDeviceToHost, s));
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));  // run_start and n_runs are reused by the next group
+        if ((int64_t)n_runs_h[gi] > E) {
+            pct_free(&proc_dev);
+            ctx->err = "DROP transposition: more runs than entries in a group";
+            return PCT_E_LIMIT;
+        }
         run_off += (int64_t)n_runs_h[gi];
     }
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
@@ -652,6 +661,10 @@ int pct_drop_prepare(pct_ctx* ctx)
                 end = rs[j];
                 break;
             }
+        if (rs[lb] >= end || end - rs[lb] > std::min<int64_t>(bk.entries, nv_index)) {
+            ctx->err = "DROP transposition: inconsistent runs for a block";
+            return PCT_E_LIMIT;
+        }
         bk.run_base += rs[lb];
         bk.n_vox = (uint32_t)(end - rs[lb]);
     }
