@@ -5,6 +5,7 @@
   head-50M      10-ellipsoid head, 50M histories, continuous angles, 256x256x96 @ ~1 mm, MSC + SM,
                 3 sigma cuts, FBP x0, 20 iterations, block 60000.
   learned-128   random ellipsoid phantoms on 128x128x64 @ 2 mm (training pairs of the warm start).
+  learned-head50m random ellipsoid phantoms with the head-50M geometry, beam and histories.
   large-200M    head, 200M histories at 230 MeV, 512x512x128 @ 0.5x0.5x1 mm, SM hull, block 120000.
 """
 from __future__ import annotations
@@ -45,6 +46,14 @@ def learned_128(seed: int, n: int = 10_000_000):
     ph = synth.phantom_random(seed, 6 + seed % 9, 10.0, q.recon_cyl_height / 2)
     bm = synth.beam(energy_mev=200.0, t_half_range=13.0, v_half_range=6.0, continuous_angles=True,
                     wepl_noise_sigma=0.25, outlier_fraction=0.01)
+    return q, ph, bm, n
+
+
+def learned_head50m(seed: int, n: int = 50_000_000):
+    """Training phantoms for a warm start at the head-50M scale: the head-50M parameters and beam with a
+    random ellipsoid phantom (6-14 ellipsoids, RSP 0-1.8)."""
+    q, _, bm, n = head_50m(n)
+    ph = synth.phantom_random(seed, 6 + seed % 9, 10.0, q.recon_cyl_height / 2)
     return q, ph, bm, n
 
 

@@ -31,9 +31,21 @@ from .learned import WarmStartNet
 from .recon import Reconstruction
 
 
-def _prepare(seed: int, n: int):
-    P, ph, bm, n = configs.learned_128(seed, n)
-    d = synth.generate(ph, bm, seed, n)
+HEAD_SEED = 2024  # data seed of the head-50M benchmark
+
+
+def _config(name: str, seed: int, n: int):
+    """Parameters, phantom, beam of phantom `seed`; seed < 0 is the head-50M head phantom itself."""
+    if name == "head-50M":
+        if seed < 0:
+            return configs.head_50m(n)
+        return configs.learned_head50m(seed, n)
+    return configs.learned_128(seed, n)
+
+
+def _prepare(seed: int, n: int, config: str = "learned-128"):
+    P, ph, bm, n = _config(config, seed, n)
+    d = synth.generate(ph, bm, HEAD_SEED if seed < 0 else seed, n)
     r = Reconstruction(P)
     r.load_histories(d["t"], d["v"], d["u"], d["wepl"], d["angle_deg"])
     del d
@@ -44,10 +56,10 @@ def _prepare(seed: int, n: int):
     return P, r
 
 
-def training_pair(seed: int, n: int, iterations: int = 40, input_iterations: int = 0):
+def training_pair(seed: int, n: int, iterations: int = 40, input_iterations: int = 0, config: str = "learned-128"):
     """(net input, hull, DROP image after `iterations` from the FBP start) of phantom `seed`; the input
     is the FBP image, or the iterate after `input_iterations` DROP iterations."""
-    P, r = _prepare(seed, n)
+    P, r = _prepare(seed, n, config)
     try:
         hull = r.image("hull").astype(np.uint8)
         if input_iterations > 0:
@@ -72,8 +84,8 @@ def _learned_start(net: WarmStartNet, r, m: int) -> dict:
 
 
 def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int, long_iterations: int = 40,
-             input_iterations: int = 0) -> dict:
-    P, r = _prepare(seed, n)
+             input_iterations: int = 0, config: str = "learned-128") -> dict:
+    P, r = _prepare(seed, n, config)
     try:
         hull = r.image("hull").astype(bool)
         # reference image: `long_iterations` DROP iterations from the FBP start
@@ -125,6 +137,9 @@ def evaluate(net: WarmStartNet, seed: int, n: int, ref_iterations: int, long_ite
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="learned-128", choices=["learned-128", "head-50M"],
+                    help="learned-128 (128x128x64 @ 2 mm, 10M histories) or head-50M (256x256x96 @ 1 mm; "
+                         "evaluated on the head-50M benchmark workload first, then on held-out random phantoms)")
     ap.add_argument("--train", type=int, default=400, help="training phantoms (seeds 0..train-1)")
     ap.add_argument("--test", type=int, default=20, help="held-out phantoms (seeds 100000..)")
     ap.add_argument("--histories", type=int, default=10_000_000)
@@ -139,10 +154,10 @@ def main(argv=None) -> int:
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
 
-    P, _, _, _ = configs.learned_128(0, 1)
+    P, _, _, _ = _config(args.config, 0, 1)
     net = WarmStartNet(P.columns, P.rows, P.slices, seed=12345)
-    report = {"workload": f"learned-128: random ellipsoid phantoms, {P.columns}x{P.rows}x{P.slices} @ 2 mm, "
-                          f"{args.histories} histories each, MSC hull, FBP x0",
+    report = {"workload": f"{args.config}: random ellipsoid phantoms, {P.columns}x{P.rows}x{P.slices}, "
+                          f"{args.histories} histories each, FBP x0",
               "train": args.train, "test": args.test, "target_iterations": args.target_iterations,
               "ref_iterations": args.ref_iterations, "input_iterations": args.input_iterations}
     if args.eval_only:
@@ -150,7 +165,8 @@ def main(argv=None) -> int:
     else:
         t0 = time.perf_counter()
         for s in range(args.train):
-            net.add_sample(*training_pair(s, args.histories, args.target_iterations, args.input_iterations))
+            net.add_sample(*training_pair(s, args.histories, args.target_iterations, args.input_iterations,
+                                          args.config))
             print(f"[warmstart] {s + 1}/{args.train} training pairs ({time.perf_counter() - t0:.0f} s)", flush=True)
         report["data_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()
@@ -164,10 +180,14 @@ def main(argv=None) -> int:
         if args.weights:
             net.save(args.weights)
     evals = []
-    for j in range(args.test):
-        e = evaluate(net, 100000 + j, args.histories, args.ref_iterations, input_iterations=args.input_iterations)
+    seeds = [100000 + j for j in range(args.test)]
+    if args.config == "head-50M":
+        seeds = [-1] + seeds  # the head-50M benchmark workload itself first
+    for seed in seeds:
+        e = evaluate(net, seed, args.histories, args.ref_iterations, input_iterations=args.input_iterations,
+                     config=args.config)
         evals.append(e)
-        print(f"[warmstart] test {j}: {json.dumps(e)}", flush=True)
+        print(f"[warmstart] test seed {seed}: {json.dumps(e)}", flush=True)
     sp = [e["speedup"] for e in evals if e["speedup"]]
     report["evaluations"] = evals
     report["median_speedup"] = float(np.median(sp)) if sp else None
