@@ -10,6 +10,9 @@ metric: MLP+DROP proton histories/s = reconstruction histories x DROP iterations
 The JSON line also carries the roofline of the dominant kernel (HIP-event timing of every launch of
 that kernel during one profiled iteration) and the oracle's CPU rate on a bounded sample.
 
+time_to_residual: on head-50M, the wall time of 20 FBP-start DROP iterations against the learned start
+(1 iteration, the shipped net on that iterate, DROP until the same residual), with the iteration counts.
+
 --workload large-200M-segment measures one GPU's share of the large-200M run instead: a 25M-history
 segment on the 512x512x128 grid (not the default line).
 """
@@ -63,6 +66,47 @@ def cpu_baseline(P, r, sample: int, threads: int, iterations: int, cpu_iteration
                       f"rate = n*{iterations}/(t_mlp + {iterations}*t_drop)"}
 
 
+def time_to_residual(r, P, weights, ref_iterations: int) -> dict:
+    """FBP start: `ref_iterations` DROP iterations give the residual target ||Ax - b||.  Learned start:
+    one DROP iteration, the net on that iterate (kept only if it lowers the residual), then DROP until
+    the target is reached.  Both timed from x0 with the lists built; the learned time includes the
+    net's inference and residual judgement."""
+    from pct_reconstruction_amd.learned import WarmStartNet
+    net = WarmStartNet(P.columns, P.rows, P.slices, seed=0)
+    try:
+        net.load(str(weights))
+        r.reset_iterate()
+        r.synchronize()
+        t0 = time.perf_counter()
+        r.DROP(ref_iterations)
+        r.synchronize()
+        t_fbp = time.perf_counter() - t0
+        target = r.residual()
+        r.reset_iterate()
+        r.DROP(1)
+        judge = net.apply_iterate(r)
+        k = 1
+        res = r.residual()
+        while k < 2 or (res > target and k < 2 * ref_iterations):
+            r.DROP(1)
+            k += 1
+            res = r.residual()
+        r.reset_iterate()
+        r.synchronize()
+        t0 = time.perf_counter()
+        r.DROP(1)
+        net.apply_iterate(r)
+        r.DROP(k - 1)
+        r.synchronize()
+        t_net = time.perf_counter() - t0
+    finally:
+        net.close()
+    return {"reference": f"FBP x0, {ref_iterations} DROP iterations", "residual_target": target,
+            "residual_reached": res, "iterations_fbp": ref_iterations, "iterations_learned": k,
+            "net_used": judge["used_net"], "fbp_ms": 1e3 * t_fbp, "learned_ms": 1e3 * t_net,
+            "speedup": t_fbp / t_net, "weights": weights.name}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +121,7 @@ def main() -> int:
                     help="reconstruction histories timed on the CPU (0: all of them)")
     ap.add_argument("--cpu-iterations", type=int, default=2, help="DROP iterations timed on the CPU")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the time-to-residual measurement")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -177,6 +222,14 @@ def main() -> int:
                 "frac": block_bytes / (block_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
                 "bytes_per_launch": block_bytes, "avg_launch_ms": block_ms}
 
+    # time to a fixed residual, learned x0 vs FBP x0 (head-50M only: the shipped net is trained for that grid)
+    warm = None
+    weights = ROOT / "pct_reconstruction_amd" / "weights" / "warmstart_head50m_x1.bin"
+    if not args.no_warm_start and args.workload == "head-50M" and weights.exists():
+        warm = time_to_residual(r, P, weights, P.iterations)
+        log(f"time to residual: FBP start {warm['fbp_ms']:.1f} ms, learned start {warm['learned_ms']:.1f} ms "
+            f"({warm['iterations_learned']} iterations), speedup {warm['speedup']:.2f}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -186,7 +239,7 @@ def main() -> int:
 
     stages = {k: round(v, 3) for k, v in m["stage_ms"].items()}
     out = {
-        "metric": "MLP+DROP proton histories/s",
+        "metric": "MLP+DROP proton histories/s; time to fixed residual: learned x0 vs FBP x0",
         "value": value,
         "unit": "histories/s",
         "n_gpus": 1,
@@ -203,6 +256,7 @@ def main() -> int:
                    "drop_iterations": iters, "drop_block_size": P.drop_block_size},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "time_to_residual": warm,
         "kernels": kernels,
         "stages_ms": stages,
         "preprocessing_s": t_pre,
