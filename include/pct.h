@@ -312,6 +312,9 @@ int pct_write_projection(const char* path, int64_t n, float angle_deg, float ene
 int pct_write_image_text(const char* path, const float* image, int columns, int rows, int slices);
 /* Binary image of write_PNG (DS.cu:9939): float32 value/2 in x-fastest order. */
 int pct_write_image_dat(const char* path, const float* image, int columns, int rows, int slices);
+/* The PNG write_PNG makes from that file with ImageMagick (DS.cu:9969-9971): 16-bit grayscale,
+ * COLUMNS x ROWS*SLICES, sample = round(clamp(value/2, 0, 1) * 65535). */
+int pct_write_image_png(const char* path, const float* image, int columns, int rows, int slices);
 
 /* ---------------------------------------------------------------- synthetic data ---- */
 /* Phantom made of up to 16 objects; each is a cylinder (kind 0, axis z) or an ellipsoid (kind 1)

@@ -150,6 +150,7 @@ def lib() -> C.CDLL:
         "pct_write_projection": ([C.c_char_p, i64, C.c_float, C.c_float, fptr4, fptr4, fptr4, vp, vp], C.c_int),
         "pct_write_image_text": ([C.c_char_p, vp, C.c_int, C.c_int, C.c_int], C.c_int),
         "pct_write_image_dat": ([C.c_char_p, vp, C.c_int, C.c_int, C.c_int], C.c_int),
+        "pct_write_image_png": ([C.c_char_p, vp, C.c_int, C.c_int, C.c_int], C.c_int),
         "pct_phantom_cylinder_c1": ([P(Phantom)], None),
         "pct_phantom_head": ([P(Phantom)], None),
         "pct_phantom_random": ([P(Phantom), C.c_uint64, C.c_int, dbl, dbl], None),

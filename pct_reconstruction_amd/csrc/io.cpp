@@ -7,7 +7,9 @@
 //   three length-prefixed strings (phantom name, data source, prepared by),
 //   then 13 arrays of N floats: t0 t1 t2 t3 v0 v1 v2 v3 u0 u1 u2 u3 WEPL,
 //   and for version 1 (and continuous-angle PCTN data) N per-history angles (float, degrees).
-// Images: array_2_disk text layout (DS.cu:10743-10787) and write_PNG's .dat (DS.cu:9939-9973).
+// Images: array_2_disk text layout (DS.cu:10743-10787) and write_PNG's .dat (DS.cu:9939-9973), plus the PNG
+// that write_PNG makes from the .dat with ImageMagick (DS.cu:9969-9971).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -119,4 +121,102 @@ extern "C" int pct_write_image_dat(const char* path, const float* image, int col
     std::vector<float> buf(nv);
     for (int64_t i = 0; i < nv; i++) buf[i] = image[i] / 2;
     return std::fwrite(buf.data(), sizeof(float), nv, f) == (size_t)nv ? PCT_OK : PCT_E_IO;
+}
+
+// write_PNG's PNG (DS.cu:9969-9971): ImageMagick `convert -define quantum:format=floating-point -depth 32
+// -size COLUMNS x ROWS*SLICES gray:file.dat file.png` reads the .dat floats (value/2) as gray levels with
+// 1.0 = white and writes a 16-bit grayscale PNG.  Here: the same COLUMNS x ROWS*SLICES image, sample =
+// round(clamp(value/2, 0, 1) * 65535), written directly (filter 0, zlib stream of stored deflate blocks).
+namespace {
+uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n)
+{
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; k++) v = (v & 1) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
+            table[i] = v;
+        }
+        ready = true;
+    }
+    for (size_t i = 0; i < n; i++) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
+    return c;
+}
+void put_be32(std::vector<uint8_t>* o, uint32_t v)
+{
+    for (int s = 24; s >= 0; s -= 8) o->push_back((uint8_t)(v >> s));
+}
+bool write_chunk(FILE* f, const char type[4], const std::vector<uint8_t>& data)
+{
+    std::vector<uint8_t> head;
+    put_be32(&head, (uint32_t)data.size());
+    head.insert(head.end(), type, type + 4);
+    uint32_t crc = crc32_update(0xFFFFFFFFu, (const uint8_t*)type, 4);
+    crc = crc32_update(crc, data.data(), data.size()) ^ 0xFFFFFFFFu;
+    std::vector<uint8_t> tail;
+    put_be32(&tail, crc);
+    return std::fwrite(head.data(), 1, head.size(), f) == head.size() &&
+           (data.empty() || std::fwrite(data.data(), 1, data.size(), f) == data.size()) &&
+           std::fwrite(tail.data(), 1, tail.size(), f) == tail.size();
+}
+}  // namespace
+
+extern "C" int pct_write_image_png(const char* path, const float* image, int columns, int rows, int slices)
+{
+    if (!path || !image || columns <= 0 || rows <= 0 || slices <= 0) return PCT_E_ARG;
+    const int64_t height = (int64_t)rows * slices;
+    if (height > 0x7FFFFFFF || columns > 0x3FFFFFFF) return PCT_E_LIMIT;
+    // raw scanlines: filter byte 0, then big-endian 16-bit samples
+    const size_t line = 1 + 2 * (size_t)columns;
+    std::vector<uint8_t> raw(line * (size_t)height);
+    for (int64_t y = 0; y < height; y++) {
+        uint8_t* p = raw.data() + (size_t)y * line;
+        *p++ = 0;
+        for (int x = 0; x < columns; x++) {
+            float v = image[x + y * columns] / 2;
+            v = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;  // NaN -> 0
+            const uint16_t q = (uint16_t)(v * 65535.0f + 0.5f);
+            *p++ = (uint8_t)(q >> 8);
+            *p++ = (uint8_t)(q & 0xFF);
+        }
+    }
+    // zlib stream: header, stored deflate blocks of at most 65535 bytes, Adler-32
+    std::vector<uint8_t> z;
+    z.reserve(raw.size() + raw.size() / 65535 * 5 + 16);
+    z.push_back(0x78);
+    z.push_back(0x01);
+    size_t pos = 0;
+    do {
+        const size_t len = std::min<size_t>(65535, raw.size() - pos);
+        const bool last = pos + len == raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back((uint8_t)(len & 0xFF));
+        z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)(~len & 0xFF));
+        z.push_back((uint8_t)((~len >> 8) & 0xFF));
+        z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + len);
+        pos += len;
+    } while (pos < raw.size());
+    uint32_t a = 1, b = 0;
+    for (uint8_t c : raw) {
+        a = (a + c) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    put_be32(&z, (b << 16) | a);
+    std::vector<uint8_t> ihdr;
+    put_be32(&ihdr, (uint32_t)columns);
+    put_be32(&ihdr, (uint32_t)height);
+    ihdr.push_back(16);  // bit depth
+    ihdr.push_back(0);   // grayscale
+    ihdr.push_back(0);   // deflate
+    ihdr.push_back(0);   // filter method 0
+    ihdr.push_back(0);   // no interlace
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return PCT_E_IO;
+    FileCloser fc{f};
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+    if (std::fwrite(sig, 1, 8, f) != 8) return PCT_E_IO;
+    if (!write_chunk(f, "IHDR", ihdr) || !write_chunk(f, "IDAT", z) || !write_chunk(f, "IEND", {})) return PCT_E_IO;
+    return PCT_OK;
 }

@@ -53,3 +53,10 @@ def write_image_dat(path: str | Path, image: np.ndarray) -> None:
     img = np.ascontiguousarray(image, np.float32)
     s, r, c = img.shape
     check(lib().pct_write_image_dat(str(path).encode(), ptr(img), c, r, s), what=f"write {path}")
+
+
+def write_image_png(path: str | Path, image: np.ndarray) -> None:
+    """write_PNG's PNG (DS.cu:9969): 16-bit gray, COLUMNS x ROWS*SLICES, round(clamp(value/2, 0, 1) * 65535)."""
+    img = np.ascontiguousarray(image, np.float32)
+    s, r, c = img.shape
+    check(lib().pct_write_image_png(str(path).encode(), ptr(img), c, r, s), what=f"write {path}")

@@ -79,6 +79,23 @@ def test_image_writers(tmp_path):
     assert np.array_equal(np.fromfile(tmp_path / "x.dat", np.float32), (img / 2).reshape(-1))
 
 
+def test_png_writer_decodes_to_the_quantized_image(tmp_path):
+    from PIL import Image
+    rng = np.random.default_rng(4)
+    img = (rng.random((3, 70, 300)) * 2.4 - 0.2).astype(np.float32)  # values below 0 and above 2 clamp
+    img[0, 0, :4] = [0.0, 2.0, 1.0, np.nan]
+    io.write_image_png(tmp_path / "x.png", img)
+    with Image.open(tmp_path / "x.png") as im:
+        assert im.size == (300, 3 * 70)  # COLUMNS x ROWS*SLICES
+        got = np.asarray(im).astype(np.int64)
+    v = np.nan_to_num(img.reshape(3 * 70, 300) / 2, nan=0.0)
+    expect = np.floor(np.clip(v, 0.0, 1.0).astype(np.float32) * np.float32(65535) + np.float32(0.5)).astype(np.int64)
+    assert np.array_equal(got, expect)
+    assert list(got[0, :4]) == [0, 65535, 32768, 0]
+    # more than one 65535-byte stored deflate block: 210 rows of 601 bytes
+    assert (tmp_path / "x.png").stat().st_size > 65535
+
+
 def test_generator_is_deterministic_and_physical():
     ph = synth.phantom_cylinder()
     a = synth.generate(ph, synth.beam(), 9, 5000)
