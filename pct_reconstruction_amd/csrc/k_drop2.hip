@@ -44,15 +44,21 @@ __device__ __forceinline__ uint32_t slab_index(uint32_t per)
     return per ? (blockIdx.x % kXcds) * per + blockIdx.x / kXcds : blockIdx.x;
 }
 
-// list entries (all `n` of the CSR capacity; unused slots hold no voxel and are mapped harmlessly).
-// Grid-stride: the capacity can exceed 2^32 entries, more work-items than one dispatch can hold.
-__global__ void k_bricks_list(int64_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, BrickGeom g,
-                              uint32_t nv)
+// The lists' entries as brick indices: the cnt[h] entries of every history h (16 lanes per history;
+// capacity slots past cnt are never read).  Grid-stride: lists can hold more entries than one
+// dispatch has work-items.
+__global__ void k_bricks_list(int64_t n, const int64_t* __restrict__ off, const uint32_t* __restrict__ cnt,
+                              const uint32_t* __restrict__ in, uint32_t* __restrict__ out, BrickGeom g, uint32_t nv)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t v = in[i];
-        out[i] = v < nv ? brick_of(v, g) : 0u;
+    const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 4;
+    const uint32_t l = threadIdx.x & 15;
+    for (int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; h < n; h += stride) {
+        const int64_t o = off[h];
+        const uint32_t c = cnt[h];
+        for (uint32_t j = l; j < c; j += 16) {
+            const uint32_t v = in[o + j];
+            out[o + j] = v < nv ? brick_of(v, g) : 0u;
+        }
     }
 }
 
@@ -562,9 +568,9 @@ int pct_drop_prepare(pct_ctx* ctx)
         }
         const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
         if (!D.vox_b_ready && ctx->list_cap > 0)  // lists built without the brick index
-            hipLaunchKernelGGL(k_bricks_list, dim3(std::min(grid_for(ctx->list_cap), 1u << 20)), dim3(kBlock), 0, s,
-                               ctx->list_cap,
-                               ctx->list_vox, D.vox_b, bg, (uint32_t)P.num_voxels);
+            hipLaunchKernelGGL(k_bricks_list, dim3(std::min(grid_for(ctx->n_recon * 16), 1u << 20)), dim3(kBlock), 0,
+                               s, ctx->n_recon, ctx->list_off, ctx->list_cnt, ctx->list_vox, D.vox_b, bg,
+                               (uint32_t)P.num_voxels);
         D.vox_b_ready = true;
         lvox = D.vox_b;
     }
