@@ -45,8 +45,17 @@ enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1, PCT_ACCUM_KAHAN32 
  * of those histories (row kernel) and of the touched voxels (column kernel); BRICKS keeps the image
  * in 4x4x2-voxel bricks (one 128-byte line each) during DROP, so consecutive voxels of a path and
  * neighbouring paths share cache lines; COLS_LDS runs the per-voxel sums with one wavefront per 64
- * voxels that stages their entries' updates in LDS with coalesced loads. */
-enum pct_drop_flags { PCT_DROP_SPATIAL_ORDER = 1, PCT_DROP_XCD_SLABS = 2, PCT_DROP_BRICKS = 4, PCT_DROP_COLS_LDS = 8 };
+ * voxels that stages their entries' updates in LDS with coalesced loads; COLS_SLICED stores the
+ * ranks of each wavefront's 64 voxels column-major (entry m of the 64 voxels contiguous, padded to
+ * the largest count), so each rank load of the per-voxel sums is one coalesced read.  COLS_LDS takes
+ * precedence over COLS_SLICED. */
+enum pct_drop_flags {
+    PCT_DROP_SPATIAL_ORDER = 1,
+    PCT_DROP_XCD_SLABS = 2,
+    PCT_DROP_BRICKS = 4,
+    PCT_DROP_COLS_LDS = 8,
+    PCT_DROP_COLS_SLICED = 16
+};
 
 /*
  * Parameters.  Field names follow the settings.cfg keys (settings.cfg:1-164) and the

@@ -21,8 +21,8 @@ HULL_IMPORT, HULL_SC, HULL_MSC, HULL_SM, HULL_FBP = 0, 1, 2, 3, 4
 FILTER_RAM_LAK, FILTER_SHEPP_LOGAN, FILTER_NONE = 0, 1, 2
 X0_IMPORT, X0_HULL, X0_FBP, X0_HYBRID, X0_ZEROS = 0, 1, 2, 3, 4
 ACCUM_FP32, ACCUM_FP64, ACCUM_KAHAN32 = 0, 1, 2
-DROP_SPATIAL_ORDER, DROP_XCD_SLABS, DROP_BRICKS, DROP_COLS_LDS = 1, 2, 4, 8
-DROP_DEFAULT_FLAGS = DROP_SPATIAL_ORDER | DROP_BRICKS  # the context's default (internal.h)
+DROP_SPATIAL_ORDER, DROP_XCD_SLABS, DROP_BRICKS, DROP_COLS_LDS, DROP_COLS_SLICED = 1, 2, 4, 8, 16
+DROP_DEFAULT_FLAGS = DROP_SPATIAL_ORDER | DROP_BRICKS | DROP_COLS_SLICED  # the context's default (internal.h)
 
 IMG = {"x": 0, "fbp": 1, "x0": 2, "hull": 3, "sc_hull": 4, "msc_counts": 5, "sm_counts": 6, "msc_hull": 7,
        "sm_hull": 8, "fbp_hull": 9, "fbp_median": 10, "hull_filtered": 11}

@@ -119,7 +119,7 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
     }
     if (const char* e = std::getenv("PCT_DROP_FLAGS")) {
         int f = std::atoi(e);
-        if (f >= 0 && f <= 15) ctx->drop_flags = f;
+        if (f >= 0 && f <= 31) ctx->drop_flags = f;
     }
     if (const char* e = std::getenv("PCT_DROP_GROUP_ENTRIES")) {
         long long g = std::atoll(e);
@@ -627,7 +627,8 @@ extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_vox
     if (!ctx) return PCT_E_ARG;
     const int g = drop_row_lanes, v = drop_col_voxels;
     if (!(g == 4 || g == 8 || g == 16 || g == 32 || g == 64) || !(v == 1 || v == 2 || v == 4 || v == 8)) return PCT_E_ARG;
-    if (drop_flags & ~(PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS | PCT_DROP_BRICKS | PCT_DROP_COLS_LDS))
+    if (drop_flags &
+        ~(PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS | PCT_DROP_BRICKS | PCT_DROP_COLS_LDS | PCT_DROP_COLS_SLICED))
         return PCT_E_ARG;
     ctx->drop_row_lanes = g;
     ctx->drop_col_voxels = v;

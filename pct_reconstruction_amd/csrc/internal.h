@@ -64,11 +64,22 @@ struct RowDesc {
 // each history through the voxel, in ascending history order).  Blocks are transposed in groups
 // (one sort per group by (block, voxel)): a block's runs start at run_base in t_vox / t_off, and
 // t_off counts entries from its group's first entry, rank_base in t_hist.
+// Sliced columns (PCT_DROP_COLS_SLICED): a block's runs in slices of 64 (the runs one column
+// wavefront takes); slice s holds the ranks of its runs column-major, entry m of run 64 j + l at
+// sell[sell_off[s] + 64 m + l], in 64 x (largest entry count of the slice) slots.
+struct SliceBlock {
+    int64_t slice_base;  // first slice of the block (the entry after the last block: total slices)
+    int64_t run;         // absolute index of the block's first run in t_vox / t_off
+    int64_t entry_base;  // the block's group's first entry in t_hist
+    uint32_t n_vox, pad;
+};
+
 struct DropPlan {
     struct Block {
         int64_t start, proc, entry_base, entries;
         int64_t group, rank_base, run_base;  // run_base: absolute index of the block's first run
         uint32_t n_vox;                      // touched voxels
+        int64_t slice_base;                  // first slice (sliced columns)
     };
     struct Group {
         int64_t first_block, n_blocks, entry_base, entries, hist_begin, hist_end;
@@ -87,6 +98,11 @@ struct DropPlan {
     uint32_t *hkey = nullptr, *hkey_sorted = nullptr, *hidx = nullptr, *hidx_sorted = nullptr;
     uint32_t *lc = nullptr, *pos = nullptr;
     int32_t* run_start = nullptr;  // per block of a group: index of its first run, -1 if none
+    uint32_t* sell = nullptr;      // sliced columns: ranks, column-major per slice
+    uint64_t* sell_off = nullptr;  // per slice (+1): first slot in sell
+    uint64_t* slice_w = nullptr;   // per slice (+1): slots (scratch of the build)
+    SliceBlock* slice_blocks = nullptr;
+    int64_t cap_sell = 0, cap_slices = 0, cap_slice_w = 0, cap_slice_blocks = 0;
     // brick layout (PCT_DROP_BRICKS): list entries and image in 4x4x2 bricks of 32 voxels
     uint32_t* vox_b = nullptr;     // list entries as brick indices (same CSR layout as list_vox)
     float* x_b = nullptr;          // image in brick order during DROP
@@ -179,7 +195,7 @@ struct pct_ctx {
     DropPlan plan;
     int drop_row_lanes = 16;   // lanes per history in the row kernel (PCT_DROP_ROW_LANES: 4..64)
     int drop_col_voxels = 1;   // voxels per lane in the column kernel (PCT_DROP_COL_VOXELS: 1, 2, 4, 8)
-    int drop_flags = PCT_DROP_SPATIAL_ORDER | PCT_DROP_BRICKS;  // PCT_DROP_FLAGS
+    int drop_flags = PCT_DROP_SPATIAL_ORDER | PCT_DROP_BRICKS | PCT_DROP_COLS_SLICED;  // PCT_DROP_FLAGS
     int64_t drop_group_entries = 1ll << 30;  // list entries per transposition group (PCT_DROP_GROUP_ENTRIES)
     double* accum64 = nullptr;
     float* accum32 = nullptr;

@@ -187,6 +187,74 @@ __global__ void k_group_runs(const uint32_t* __restrict__ run_keys, const uint32
     if (lb < n_blocks && (u == 0 || (run_keys[u - 1] >> vbits) != lb)) run_start[lb] = (int32_t)u;
 }
 
+// ---- sliced columns (once per list build, after the transposition) ----
+// Slice s of all blocks' slices belongs to the block k with sb[k].slice_base <= s < sb[k + 1].slice_base
+// (sb holds nb + 1 entries, the last one the total).
+__device__ __forceinline__ int64_t slice_block(const SliceBlock* __restrict__ sb, int64_t nb, int64_t s)
+{
+    int64_t lo = 0, hi = nb;  // sb[lo].slice_base <= s < sb[hi].slice_base
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (sb[mid].slice_base <= s) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// per slice (one wavefront, grid-stride over slices): 64 x the largest entry count of its runs
+__global__ void __launch_bounds__(kWave) k_slice_width(int64_t NS, int64_t nb, const SliceBlock* __restrict__ sb,
+                                                        const uint32_t* __restrict__ t_off, uint64_t* __restrict__ w)
+{
+    for (int64_t s = blockIdx.x; s < NS; s += gridDim.x) {
+        const int64_t k = slice_block(sb, nb, s);
+        const int64_t j = (s - sb[k].slice_base) * kWave + threadIdx.x;
+        uint32_t c = 0;
+        if (j < (int64_t)sb[k].n_vox) {
+            const int64_t u = sb[k].run + j;
+            c = t_off[u + 1] - t_off[u];
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const uint32_t y = (uint32_t)__shfl_xor((int)c, o, kWave);
+            c = y > c ? y : c;
+        }
+        if (threadIdx.x == 0) w[s] = (uint64_t)c * kWave;
+    }
+}
+
+// per slice: lane l copies the ranks of the slice's run l to slots sell_off[s] + l + 64 m.  The
+// slice's entries are one contiguous range of t_hist; up to kFillStage of them are first loaded
+// into LDS with coalesced reads, so that both the reads and the column-major writes coalesce.
+constexpr int kFillStage = 2048;
+
+__global__ void __launch_bounds__(kWave) k_slice_fill(int64_t NS, int64_t nb, const SliceBlock* __restrict__ sb,
+                                                       const uint32_t* __restrict__ t_off,
+                                                       const uint32_t* __restrict__ t_hist,
+                                                       const uint64_t* __restrict__ sell_off, uint32_t* __restrict__ sell)
+{
+    __shared__ uint32_t stage[kFillStage];
+    const int lane = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < NS; s += gridDim.x) {
+        const int64_t k = slice_block(sb, nb, s);
+        const int64_t j0 = (s - sb[k].slice_base) * kWave;
+        const int64_t left = (int64_t)sb[k].n_vox - j0;
+        const int n_runs = left < kWave ? (int)left : kWave;  // >= 1
+        const int64_t u0 = sb[k].run + j0;
+        const uint32_t E0 = t_off[u0], E1 = t_off[u0 + n_runs];
+        const uint32_t b = lane < n_runs ? t_off[u0 + lane] : E1, c = lane < n_runs ? t_off[u0 + lane + 1] - b : 0u;
+        const uint32_t* __restrict__ src = t_hist + sb[k].entry_base;
+        uint32_t* __restrict__ dst = sell + sell_off[s] + lane;
+        if (E1 - E0 <= (uint32_t)kFillStage) {
+            for (uint32_t i = lane; i < E1 - E0; i += kWave) stage[i] = src[E0 + i];
+            __syncthreads();
+            for (uint32_t m = 0; m < c; m++) dst[(int64_t)m * kWave] = stage[b - E0 + m];
+            __syncthreads();
+        } else {
+            for (uint32_t m = 0; m < c; m++) dst[(int64_t)m * kWave] = src[b + m];
+        }
+    }
+}
+
 // per history: u_i (DS.cu:9321-9380 with the list of find_MLP_path_GPU_tabulated).
 // G lanes per history (G | 64), histories taken in rank order: lane l sums entries l, l+G, l+2G, ...
 // with kUnroll list loads and then kUnroll gathers in flight, then the G partial sums are combined
@@ -318,6 +386,70 @@ __global__ void __launch_bounds__(kBlock) k_drop_cols(uint32_t U, uint32_t per, 
         if (e[q] > b[q]) x[v[q]] = xv[q] + col_update<ACC, K>(b[q], e[q], rv[q], t_rank, r);
 }
 
+// col_update over a sliced column: entry m's rank at sp[64 m].
+template <int ACC, int K>
+__device__ __forceinline__ float col_update_sliced(uint32_t n, const double* rv, const uint32_t* __restrict__ sp,
+                                                   const double* __restrict__ r)
+{
+    ColAcc<ACC> a;
+#pragma unroll
+    for (int m = 0; m < K; m++)
+        if ((uint32_t)m < n) a.add(rv[m]);
+    for (uint32_t c0 = K; c0 < n; c0 += K) {
+        uint32_t rk[K];
+        double rw[K];
+#pragma unroll
+        for (int m = 0; m < K; m++) rk[m] = c0 + m < n ? sp[(int64_t)(c0 + m) * kWave] : 0u;
+#pragma unroll
+        for (int m = 0; m < K; m++) rw[m] = c0 + m < n ? r[rk[m]] : 0.0;
+#pragma unroll
+        for (int m = 0; m < K; m++)
+            if (c0 + m < n) a.add(rw[m]);
+    }
+    return a.mean(n);
+}
+
+// k_drop_cols over sliced columns (PCT_DROP_COLS_SLICED): the same sums in the same order, with
+// each rank load one coalesced 256-byte read of the wavefront's slice.  sell_off points at the
+// block's first slice; lane l of the wavefront that takes runs [64 j, 64 j + 64) reads slice j.
+template <int ACC, int V>
+__global__ void __launch_bounds__(kBlock) k_drop_cols_sliced(uint32_t U, uint32_t per,
+                                                              const uint32_t* __restrict__ t_vox,
+                                                              const uint32_t* __restrict__ t_off,
+                                                              const uint64_t* __restrict__ sell_off,
+                                                              const uint32_t* __restrict__ sell,
+                                                              const double* __restrict__ r, float* __restrict__ x)
+{
+    constexpr int K = V <= 2 ? 8 : 4;
+    const uint32_t u0 = slab_index(per) * (uint32_t)(kBlock * V) + threadIdx.x;
+    uint32_t n[V], v[V], rk[V][K];
+    const uint32_t* sp[V];
+    float xv[V];
+    double rv[V][K];
+#pragma unroll
+    for (int q = 0; q < V; q++) {
+        const uint32_t u = u0 + q * kBlock;
+        const bool ok = u < U;
+        const uint32_t b = ok ? t_off[u] : 0u, e = ok ? t_off[u + 1] : 0u;
+        n[q] = e - b;
+        v[q] = ok ? t_vox[u] : 0u;
+        sp[q] = sell + (ok ? sell_off[u / kWave] : 0ull) + (u % kWave);
+    }
+#pragma unroll
+    for (int q = 0; q < V; q++)
+#pragma unroll
+        for (int m = 0; m < K; m++) rk[q][m] = (uint32_t)m < n[q] ? sp[q][m * kWave] : 0u;
+#pragma unroll
+    for (int q = 0; q < V; q++) {
+        xv[q] = n[q] > 0 ? x[v[q]] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < K; m++) rv[q][m] = (uint32_t)m < n[q] ? r[rk[q][m]] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < V; q++)
+        if (n[q] > 0) x[v[q]] = xv[q] + col_update_sliced<ACC, K>(n[q], rv[q], sp[q], r);
+}
+
 // Column update with the entries staged per wavefront (PCT_DROP_COLS_LDS).  A one-wave workgroup
 // takes 64 consecutive touched voxels; their entries are one contiguous range of t_rank.  All 64
 // lanes load that range coalesced and gather the updates r into LDS (kStage entries per round);
@@ -393,6 +525,48 @@ int grow(pct_ctx* ctx, T** p, int64_t* cap, int64_t need)
     return PCT_OK;
 }
 
+// Sliced columns of every block (PCT_DROP_COLS_SLICED), from the transposed runs: slice widths,
+// their prefix sum (slots), then the ranks copied column-major.  Sets each block's slice_base.
+int build_slices(pct_ctx* ctx, DropPlan& D)
+{
+    hipStream_t s = ctx->stream;
+    const int64_t nb = (int64_t)D.blocks.size();
+    std::vector<SliceBlock> sb(nb + 1);
+    int64_t NS = 0;
+    for (int64_t k = 0; k < nb; k++) {
+        DropPlan::Block& bk = D.blocks[k];
+        bk.slice_base = NS;
+        sb[k] = SliceBlock{NS, bk.run_base, bk.rank_base, bk.n_vox, 0u};
+        NS += ((int64_t)bk.n_vox + kWave - 1) / kWave;
+    }
+    sb[nb] = SliceBlock{NS, 0, 0, 0u, 0u};
+    PCT_TRY(grow(ctx, &D.slice_blocks, &D.cap_slice_blocks, nb + 1));
+    PCT_TRY(grow(ctx, &D.sell_off, &D.cap_slices, NS + 1));
+    PCT_TRY(grow(ctx, &D.slice_w, &D.cap_slice_w, NS + 1));
+    uint64_t* w = D.slice_w;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(D.slice_blocks, sb.data(), sb.size() * sizeof(SliceBlock), hipMemcpyHostToDevice, s));
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(w + NS, 0, sizeof(uint64_t), s));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(NS, 1 << 20));
+    if (NS > 0) hipLaunchKernelGGL(k_slice_width, dim3(grid), dim3(kWave), 0, s, NS, nb, D.slice_blocks, D.t_off, w);
+    size_t tb = 0;
+    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w, D.sell_off, (int)(NS + 1), s));
+    if (tb > ctx->cub_tmp_bytes) {
+        PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, tb));
+        ctx->cub_tmp_bytes = tb;
+    }
+    tb = ctx->cub_tmp_bytes;
+    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, w, D.sell_off, (int)(NS + 1), s));
+    uint64_t total = 0;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&total, D.sell_off + NS, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    PCT_TRY(grow(ctx, &D.sell, &D.cap_sell, (int64_t)total + 1));
+    if (NS > 0)
+        hipLaunchKernelGGL(k_slice_fill, dim3(grid), dim3(kWave), 0, s, NS, nb, D.slice_blocks, D.t_off, D.t_hist,
+                           D.sell_off, D.sell);
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
 }  // namespace
 
 void pct_drop_plan_free(DropPlan* D)
@@ -416,6 +590,11 @@ void pct_drop_plan_free(DropPlan* D)
     pct_free(&D->lc);
     pct_free(&D->pos);
     pct_free(&D->run_start);
+    pct_free(&D->sell);
+    pct_free(&D->sell_off);
+    pct_free(&D->slice_w);
+    pct_free(&D->slice_blocks);
+    D->cap_sell = D->cap_slices = D->cap_slice_w = D->cap_slice_blocks = 0;
     pct_free(&D->vox_b);
     pct_free(&D->x_b);
     D->cap_vox_b = D->cap_x_b = 0;
@@ -674,6 +853,8 @@ int pct_drop_prepare(pct_ctx* ctx)
         bk.run_base += rs[lb];
         bk.n_vox = (uint32_t)(end - rs[lb]);
     }
+    if ((ctx->drop_flags & PCT_DROP_COLS_SLICED) && !(ctx->drop_flags & PCT_DROP_COLS_LDS))
+        PCT_TRY(build_slices(ctx, D));
     D.flags = ctx->drop_flags;
     D.ready = true;
     (void)nb;
@@ -753,6 +934,17 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
     default: PCT_COLS(A, 4); break;    \
     }
             const dim3 lg((U + kWave - 1) / kWave), lb(kWave);
+            const uint64_t* sell_off = D.sell_off + bk.slice_base;
+#define PCT_COLS_S(A, VV)                                                                                       \
+    drop_launch(k_drop_cols_sliced<A, VV>, cg, blk, s, ec, U, cper, t_vox, t_off, sell_off, (const uint32_t*)D.sell, \
+                (const double*)D.r, xx)
+#define PCT_COLS_SV(A)                   \
+    switch (V) {                         \
+    case 1: PCT_COLS_S(A, 1); break;     \
+    case 2: PCT_COLS_S(A, 2); break;     \
+    case 8: PCT_COLS_S(A, 8); break;     \
+    default: PCT_COLS_S(A, 4); break;    \
+    }
             if (D.flags & PCT_DROP_COLS_LDS) {
                 if (P.drop_accum == PCT_ACCUM_FP64)
                     drop_launch(k_drop_cols_lds<PCT_ACCUM_FP64>, lg, lb, s, ec, U, t_vox, t_off, t_rank, D.r, xx);
@@ -760,6 +952,14 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
                     drop_launch(k_drop_cols_lds<PCT_ACCUM_FP32>, lg, lb, s, ec, U, t_vox, t_off, t_rank, D.r, xx);
                 else
                     drop_launch(k_drop_cols_lds<PCT_ACCUM_KAHAN32>, lg, lb, s, ec, U, t_vox, t_off, t_rank, D.r, xx);
+            } else if (D.flags & PCT_DROP_COLS_SLICED) {
+                if (P.drop_accum == PCT_ACCUM_FP64) {
+                    PCT_COLS_SV(PCT_ACCUM_FP64)
+                } else if (P.drop_accum == PCT_ACCUM_FP32) {
+                    PCT_COLS_SV(PCT_ACCUM_FP32)
+                } else {
+                    PCT_COLS_SV(PCT_ACCUM_KAHAN32)
+                }
             } else if (P.drop_accum == PCT_ACCUM_FP64) {
                 PCT_COLS_V(PCT_ACCUM_FP64)
             } else if (P.drop_accum == PCT_ACCUM_FP32) {
@@ -767,6 +967,8 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
             } else {
                 PCT_COLS_V(PCT_ACCUM_KAHAN32)
             }
+#undef PCT_COLS_SV
+#undef PCT_COLS_S
 #undef PCT_COLS_V
 #undef PCT_COLS
         }

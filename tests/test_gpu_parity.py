@@ -96,12 +96,14 @@ def _run_and_compare(P, data, oracle, case):
         r.reset_iterate()
         r.DROP(P.iterations)
         assert _mismatch(r.image("x").reshape(-1), x) == 0
-        # scheduling (visiting order, XCD slabs, voxels per lane) never changes a bit; the lanes per
+        # scheduling (visiting order, XCD slabs, voxels per lane, column layout) never changes a bit; the lanes per
         # history only regroup an FP64 sum of floats
-        for shape in [(16, 1, False, False, False, False), (16, 8, True, False, True, False),
-                      (16, 2, False, True, False, True), (16, 1, True, True, True, True),
-                      (16, 1, True, False, True, True), (64, 4, True, True, False, False),
-                      (4, 4, True, True, True, True)]:
+        for shape in [(16, 1, False, False, False, False, False), (16, 8, True, False, True, False, False),
+                      (16, 2, False, True, False, True, False), (16, 1, True, True, True, True, False),
+                      (16, 1, True, False, True, True, False), (64, 4, True, True, False, False, False),
+                      (4, 4, True, True, True, True, False), (16, 1, True, False, True, False, False),
+                      (16, 2, True, True, True, False, True), (16, 8, False, False, False, False, True),
+                      (16, 4, True, False, True, False, True), (16, 1, True, False, True, True, True)]:
             r.set_launch(*shape)
             r.reset_iterate()
             r.DROP(P.iterations)

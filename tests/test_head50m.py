@@ -74,9 +74,10 @@ def test_head_50m_full_size_parity_and_invariants(oracle):
         r.reset_iterate()
         r.DROP(3)
         assert np.array_equal(r.image("x").reshape(-1), x3)
-        r.set_launch(spatial_order=False, bricks=False)
-        r.reset_iterate()
-        r.DROP(3)
-        assert np.array_equal(r.image("x").reshape(-1), x3)
+        for launch in (dict(spatial_order=False, bricks=False), dict(cols_sliced=False)):
+            r.set_launch(**launch)
+            r.reset_iterate()
+            r.DROP(3)
+            assert np.array_equal(r.image("x").reshape(-1), x3), launch
         print(f"[head-50M] DROP residuals {[round(x, 1) for x in res]}; reruns and schedules bit-identical",
               flush=True)
