@@ -201,12 +201,16 @@ __device__ __forceinline__ int64_t slice_block(const SliceBlock* __restrict__ sb
     return lo;
 }
 
-// per slice (one wavefront, grid-stride over slices): 64 x the largest entry count of its runs
-__global__ void __launch_bounds__(kWave) k_slice_width(int64_t NS, int64_t nb, const SliceBlock* __restrict__ sb,
+// per slice (one wavefront; workgroup g takes slices [g chunk, (g + 1) chunk)): 64 x the largest
+// entry count of its runs
+__global__ void __launch_bounds__(kWave) k_slice_width(int64_t NS, int64_t chunk, int64_t nb,
+                                                        const SliceBlock* __restrict__ sb,
                                                         const uint32_t* __restrict__ t_off, uint64_t* __restrict__ w)
 {
-    for (int64_t s = blockIdx.x; s < NS; s += gridDim.x) {
-        const int64_t k = slice_block(sb, nb, s);
+    const int64_t s0 = (int64_t)blockIdx.x * chunk, s1 = s0 + chunk < NS ? s0 + chunk : NS;
+    int64_t k = s0 < s1 ? slice_block(sb, nb, s0) : 0;
+    for (int64_t s = s0; s < s1; s++) {
+        while (sb[k + 1].slice_base <= s) k++;
         const int64_t j = (s - sb[k].slice_base) * kWave + threadIdx.x;
         uint32_t c = 0;
         if (j < (int64_t)sb[k].n_vox) {
@@ -225,17 +229,20 @@ __global__ void __launch_bounds__(kWave) k_slice_width(int64_t NS, int64_t nb, c
 // per slice: lane l copies the ranks of the slice's run l to slots sell_off[s] + l + 64 m.  The
 // slice's entries are one contiguous range of t_hist; up to kFillStage of them are first loaded
 // into LDS with coalesced reads, so that both the reads and the column-major writes coalesce.
-constexpr int kFillStage = 2048;
+constexpr int kFillStage = 1024;
 
-__global__ void __launch_bounds__(kWave) k_slice_fill(int64_t NS, int64_t nb, const SliceBlock* __restrict__ sb,
+__global__ void __launch_bounds__(kWave) k_slice_fill(int64_t NS, int64_t chunk, int64_t nb,
+                                                       const SliceBlock* __restrict__ sb,
                                                        const uint32_t* __restrict__ t_off,
                                                        const uint32_t* __restrict__ t_hist,
                                                        const uint64_t* __restrict__ sell_off, uint32_t* __restrict__ sell)
 {
     __shared__ uint32_t stage[kFillStage];
     const int lane = threadIdx.x;
-    for (int64_t s = blockIdx.x; s < NS; s += gridDim.x) {
-        const int64_t k = slice_block(sb, nb, s);
+    const int64_t s0 = (int64_t)blockIdx.x * chunk, s1 = s0 + chunk < NS ? s0 + chunk : NS;
+    int64_t k = s0 < s1 ? slice_block(sb, nb, s0) : 0;
+    for (int64_t s = s0; s < s1; s++) {
+        while (sb[k + 1].slice_base <= s) k++;
         const int64_t j0 = (s - sb[k].slice_base) * kWave;
         const int64_t left = (int64_t)sb[k].n_vox - j0;
         const int n_runs = left < kWave ? (int)left : kWave;  // >= 1
@@ -546,8 +553,11 @@ int build_slices(pct_ctx* ctx, DropPlan& D)
     uint64_t* w = D.slice_w;
     PCT_CHECK_HIP(ctx, hipMemcpyAsync(D.slice_blocks, sb.data(), sb.size() * sizeof(SliceBlock), hipMemcpyHostToDevice, s));
     PCT_CHECK_HIP(ctx, hipMemsetAsync(w + NS, 0, sizeof(uint64_t), s));
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(NS, 1 << 20));
-    if (NS > 0) hipLaunchKernelGGL(k_slice_width, dim3(grid), dim3(kWave), 0, s, NS, nb, D.slice_blocks, D.t_off, w);
+    // contiguous chunks of slices per workgroup: one block search per workgroup
+    const int64_t chunk = std::max<int64_t>(1, (NS + 65535) / 65536);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, (NS + chunk - 1) / chunk);
+    if (NS > 0)
+        hipLaunchKernelGGL(k_slice_width, dim3(grid), dim3(kWave), 0, s, NS, chunk, nb, D.slice_blocks, D.t_off, w);
     size_t tb = 0;
     PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w, D.sell_off, (int)(NS + 1), s));
     if (tb > ctx->cub_tmp_bytes) {
@@ -561,8 +571,8 @@ int build_slices(pct_ctx* ctx, DropPlan& D)
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
     PCT_TRY(grow(ctx, &D.sell, &D.cap_sell, (int64_t)total + 1));
     if (NS > 0)
-        hipLaunchKernelGGL(k_slice_fill, dim3(grid), dim3(kWave), 0, s, NS, nb, D.slice_blocks, D.t_off, D.t_hist,
-                           D.sell_off, D.sell);
+        hipLaunchKernelGGL(k_slice_fill, dim3(grid), dim3(kWave), 0, s, NS, chunk, nb, D.slice_blocks, D.t_off,
+                           D.t_hist, D.sell_off, D.sell);
     PCT_CHECK_HIP(ctx, hipGetLastError());
     return PCT_OK;
 }
