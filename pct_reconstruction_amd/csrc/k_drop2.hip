@@ -816,17 +816,26 @@ int pct_drop_prepare(pct_ctx* ctx)
             PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, tb, D.keys, D.keys_sorted, D.vals,
                                                                   D.t_hist + g.entry_base, (int)E, 0,
                                                                   bits_for(g.n_blocks) + vbits, s));
-            PCT_CHECK_HIP(ctx, hipMemsetAsync(D.runs, 0, (E + 1) * sizeof(uint32_t), s));
             tb = ctx->cub_tmp_bytes;
             PCT_CHECK_HIP(ctx, hipcub::DeviceRunLengthEncode::Encode(ctx->cub_tmp, tb, D.keys_sorted, D.keys, D.runs,
                                                                      D.n_runs, (int)E, s));
+            // the run count sizes the scan and the runs kernel
+            PCT_CHECK_HIP(ctx, hipMemcpyAsync(&n_runs_h[gi], D.n_runs, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+            const int64_t nr = (int64_t)n_runs_h[gi];
+            if (nr < 1 || nr > E) {
+                pct_free(&proc_dev);
+                ctx->err = "DROP transposition: run count out of range in a group";
+                return PCT_E_LIMIT;
+            }
+            // t_off[u] = first entry of run u; t_off[nr] = E (the zero appended to the counts)
+            PCT_CHECK_HIP(ctx, hipMemsetAsync(D.runs + nr, 0, sizeof(uint32_t), s));
             tb = ctx->cub_tmp_bytes;
             PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.runs, D.t_off + run_off + gi,
-                                                                (int)(E + 1), s));
-            hipLaunchKernelGGL(k_group_runs, dim3(grid_for(E)), dim3(kBlock), 0, s, D.keys, D.n_runs, vbits,
+                                                                (int)(nr + 1), s));
+            hipLaunchKernelGGL(k_group_runs, dim3(grid_for(nr)), dim3(kBlock), 0, s, D.keys, D.n_runs, vbits,
                                (uint32_t)g.n_blocks, D.t_vox + run_off + gi, D.run_start);
             PCT_CHECK_HIP(ctx, hipGetLastError());
-            PCT_CHECK_HIP(ctx, hipMemcpyAsync(&n_runs_h[gi], D.n_runs, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
         run_start_h[gi].assign(g.n_blocks + 1, -1);
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(run_start_h[gi].data(), D.run_start, g.n_blocks * sizeof(int32_t),
