@@ -15,7 +15,9 @@
 //                 (FP32 as the reference, FP32 with Kahan compensation, or FP64), S = the number of
 //                 entries, x[v] += sum / S.
 // No atomics, no scratch clearing, and the per-voxel sums are formed in the same order as a
-// single-threaded run of the reference: the result is reproducible bit for bit.
+// single-threaded run of the reference: the result is reproducible bit for bit.  By default the
+// column kernel reads the ranks from a sliced copy of the transposed lists (column-major per 64
+// touched voxels, the voxels of one wavefront), so that each rank load is one coalesced read.
 //
 // Locality.  Both kernels are limited by the rate of scattered 4-8 byte accesses (x gathers in the
 // rows, r gathers in the columns), which grows with the number of distinct cache lines a wavefront
