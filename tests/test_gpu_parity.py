@@ -162,3 +162,33 @@ def test_drop_transposition_groups_do_not_change_bits(monkeypatch):
 def test_parity_drop_accumulation_modes(oracle, accum):
     P, data = small_cylinder(n=40000, seed=21, drop_accum=accum, iterations=3)
     _run_and_compare(P, data, oracle, f"cylinder-accum{accum}")
+
+
+def test_tiny_and_empty_drop_blocks(oracle):
+    """Blocks of 7 histories with MIN_MLP_LENGTH 55 (about the 70th percentile of the path lengths):
+    of the 1045 blocks, about one in five has no list entry at all and the others only a few touched
+    voxels, so the transposition groups, the run starts and the column slices meet empty and
+    one-slice blocks.  The image matches the oracle's, and every column layout gives the same bits."""
+    from pct_reconstruction_amd import Reconstruction
+    P, data = small_cylinder(n=20000, seed=8, drop_block_size=7, histories_per_block=1, iterations=2,
+                             min_mlp_length=55)
+    ref = oracle.run_pipeline(P, data, iterations=P.iterations, accum=P.drop_accum)
+    cnt = np.diff(ref["offsets"])
+    long_ = cnt >= P.min_mlp_length
+    nb = (cnt.size + 6) // 7
+    empty = sum(not long_[7 * b:7 * b + 7].any() for b in range(nb))
+    print(f"[tiny blocks] {cnt.size} histories, {nb} blocks, {empty} without a long path", flush=True)
+    assert empty > 0
+    with Reconstruction(P) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.run()
+        x = r.image("x").reshape(-1)
+        rel = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
+        assert rel <= 1e-6, rel
+        for launch in (dict(cols_sliced=False), dict(cols_lds=True), dict(drop_col_voxels=2),
+                       dict(xcd_slabs=True), dict(spatial_order=False, bricks=False)):
+            r.set_launch(**launch)
+            r.reset_iterate()
+            r.DROP(P.iterations)
+            assert _mismatch(r.image("x").reshape(-1), x) == 0, launch
+    print(f"[tiny blocks] rel L2 vs oracle {rel:.2e}; layouts bit-identical", flush=True)
