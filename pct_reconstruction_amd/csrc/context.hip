@@ -160,6 +160,7 @@ int pct_ctx_destroy(pct_ctx* ctx)
     pct_free(&ctx->fbp_x0); pct_free(&ctx->hull_endpoints);
     pct_free(&ctx->list_off); pct_free(&ctx->list_cnt); pct_free(&ctx->list_vox); pct_free(&ctx->chord);
     pct_free(&ctx->cap_scratch);
+    pct_free(&ctx->res_r2);
     pct_free(&ctx->T.sin_t); pct_free(&ctx->T.cos_t); pct_free(&ctx->T.scat); pct_free(&ctx->T.p12);
     pct_free(&ctx->T.p26); pct_free(&ctx->T.p312); pct_free(&ctx->T.step_terms);
     pct_free(&ctx->accum64); pct_free(&ctx->accum32); pct_free(&ctx->S); pct_free(&ctx->touched);

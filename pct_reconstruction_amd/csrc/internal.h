@@ -188,6 +188,8 @@ struct pct_ctx {
     uint32_t* list_vox = nullptr;
     double* chord = nullptr;       // n_recon
     int64_t* cap_scratch = nullptr;  // n_recon + 1
+    double* res_r2 = nullptr;        // residual: per history (b - a.x)^2, and their sum
+    int64_t res_cap = 0;
     int64_t list_cap = 0, list_entries = 0, list_hist_cap = 0;
     DevTables T;
 

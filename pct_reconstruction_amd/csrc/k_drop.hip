@@ -281,57 +281,3 @@ extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, d
     return PCT_OK;
 }
 
-namespace {
-__global__ void k_residual_rows(int64_t n, const int64_t* __restrict__ off, const uint32_t* __restrict__ cnt_arr,
-                                const uint32_t* __restrict__ vox, const double* __restrict__ chord,
-                                const float* __restrict__ wepl, const float* __restrict__ x, int min_len,
-                                double* __restrict__ r2)
-{
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t p = wave; p < n; p += n_waves) {
-        const int64_t base = off[p];
-        const int64_t cnt = cnt_arr[p];
-        double part = 0.0;
-        for (int64_t j = lane; j < cnt; j += kWave) part += (double)x[vox[base + j]];
-        double s = wave_sum(part);
-        if (lane == 0) {
-            double r = (double)wepl[p] - chord[p] * s;
-            r2[p] = cnt >= min_len ? r * r : 0.0;
-        }
-    }
-}
-}  // namespace
-
-#include <hipcub/hipcub.hpp>
-int pct_stage_residual(pct_ctx* ctx, double* norm)
-{
-    const int64_t n = ctx->n_recon;
-    hipStream_t s = ctx->stream;
-    double* r2 = nullptr;
-    double* out = nullptr;
-    PCT_TRY(pct_alloc(ctx, &r2, n));
-    PCT_TRY(pct_alloc(ctx, &out, 1));
-    PCT_CHECK_HIP(ctx, hipMemsetAsync(out, 0, sizeof(double), s));
-    if (n > 0) {
-        int64_t blocks = (n * kWave + kBlock - 1) / kBlock;
-        if (blocks > 8192) blocks = 8192;
-        hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)blocks), dim3(kBlock), 0, s, n, ctx->list_off, ctx->list_cnt,
-                           ctx->list_vox, ctx->chord, ctx->wepl, ctx->x, ctx->P.min_mlp_length, r2);
-        size_t need = 0;
-        PCT_CHECK_HIP(ctx, hipcub::DeviceReduce::Sum(nullptr, need, r2, out, n, s));
-        if (need > ctx->cub_tmp_bytes) {
-            PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
-            ctx->cub_tmp_bytes = need;
-        }
-        PCT_CHECK_HIP(ctx, hipcub::DeviceReduce::Sum(ctx->cub_tmp, need, r2, out, n, s));
-    }
-    double h = 0.0;
-    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s));
-    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    pct_free(&r2);
-    pct_free(&out);
-    *norm = std::sqrt(h);
-    return PCT_OK;
-}

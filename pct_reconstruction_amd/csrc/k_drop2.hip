@@ -305,6 +305,41 @@ __global__ void __launch_bounds__(kBlock) k_drop_rows(int64_t start, int64_t pro
     }
 }
 
+// per history: (b_i - L_i sum_j x[v_ij])^2 for the data residual (0 below MIN_MLP_LENGTH), with the
+// row kernel's lanes and loads: G lanes per history, kUnroll list loads and then kUnroll gathers in
+// flight per lane, a butterfly over the G partial sums.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_residual(int64_t n, const int64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ cnt_arr,
+                                                      const uint32_t* __restrict__ vox, const double* __restrict__ chord,
+                                                      const float* __restrict__ wepl, const float* __restrict__ x,
+                                                      int min_len, double* __restrict__ r2)
+{
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (p >= n) return;  // the G lanes of a history leave together
+    const uint32_t* __restrict__ lst = vox + off[p];
+    const int cnt = (int)cnt_arr[p];
+    double part = 0.0;
+    for (int j = lane; j < cnt; j += kUnroll * G) {
+        uint32_t a[kUnroll];
+        float xv[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) a[q] = j + q * G < cnt ? lst[j + q * G] : 0u;
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) xv[q] = j + q * G < cnt ? x[a[q]] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++)
+            if (j + q * G < cnt) part += (double)xv[q];
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
+    if (lane == 0) {
+        const double r = (double)wepl[p] - chord[p] * part;
+        r2[p] = cnt >= min_len ? r * r : 0.0;
+    }
+}
+
 // Running per-voxel sum in history order: FP64 (s = u_1 + u_2 + ...), FP32 (the reference's float
 // sums), or FP32 with Kahan compensation (from s = c = 0).
 template <int ACC>
@@ -1013,5 +1048,49 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
         for (hipEvent_t e : ev_rows) (void)hipEventDestroy(e);
         for (hipEvent_t e : ev_cols) (void)hipEventDestroy(e);
     }
+    return PCT_OK;
+}
+
+// ||b - A x|| over the reconstruction histories (the judge of the learned start and of the tests).
+// With the brick layout prepared, the lists' brick indices gather from a brick-ordered copy of x.
+int pct_stage_residual(pct_ctx* ctx, double* norm)
+{
+    const int64_t n = ctx->n_recon;
+    hipStream_t s = ctx->stream;
+    DropPlan& D = ctx->plan;
+    if (ctx->res_cap < n + 1 || !ctx->res_r2) {
+        PCT_TRY(pct_alloc(ctx, &ctx->res_r2, n + 1));
+        ctx->res_cap = n + 1;
+    }
+    double* out = ctx->res_r2 + n;
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(out, 0, sizeof(double), s));
+    if (n > 0) {
+        const uint32_t* vox = ctx->list_vox;
+        const float* x = ctx->x;
+        if (D.ready && (D.flags & PCT_DROP_BRICKS) && D.vox_b_ready && D.x_b) {
+            const pct_params& P = ctx->P;
+            const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
+            hipLaunchKernelGGL(k_x_bricks, dim3(grid_for(P.num_voxels)), dim3(kBlock), 0, s, (int64_t)P.num_voxels,
+                               ctx->x, D.x_b, bg, 1);
+            vox = D.vox_b;
+            x = D.x_b;
+        }
+        constexpr int G = 16;
+        hipLaunchKernelGGL(k_residual<G>, dim3(grid_for(n * G)), dim3(kBlock), 0, s, n, ctx->list_off, ctx->list_cnt,
+                           vox, ctx->chord, ctx->wepl, x, ctx->P.min_mlp_length, ctx->res_r2);
+        PCT_CHECK_HIP(ctx, hipGetLastError());
+        size_t need = 0;
+        PCT_CHECK_HIP(ctx, hipcub::DeviceReduce::Sum(nullptr, need, ctx->res_r2, out, n, s));
+        if (need > ctx->cub_tmp_bytes) {
+            PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
+            ctx->cub_tmp_bytes = need;
+        }
+        need = ctx->cub_tmp_bytes;
+        PCT_CHECK_HIP(ctx, hipcub::DeviceReduce::Sum(ctx->cub_tmp, need, ctx->res_r2, out, n, s));
+    }
+    double h = 0.0;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    *norm = std::sqrt(h);
     return PCT_OK;
 }
