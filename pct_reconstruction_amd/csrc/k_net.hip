@@ -132,6 +132,77 @@ __global__ void __launch_bounds__(kBlock) k_conv_fwd(ConvIn in, int D, int H, in
     }
 }
 
+// k_conv_fwd with the input staged in LDS.  A workgroup takes an output tile of 16 (x) x TY (y) x 2 (z)
+// voxels (TY = 4 or 2): it loads the tile's 18 x (TY + 2) x 4 input halo (all Cin channels, zeros
+// outside the grid) once, and each wave computes TY / 2 rows of 16 voxels from LDS, so every input
+// voxel is read from memory about once per tile instead of once per tap.  The k order, and so every
+// output value, is that of k_conv_fwd.  LDS: halo voxels * Cin bf16 (dynamic).
+constexpr int kTileX = 16, kTileZ = 2;
+constexpr int halo_vox(int ty) { return (kTileX + 2) * (ty + 2) * (kTileZ + 2); }
+
+template <int NT, int TY>
+__global__ void __launch_bounds__(kBlock) k_conv_fwd_tiled(ConvIn in, int D, int H, int W, int Kp,
+                                                            const uint16_t* __restrict__ Wt,
+                                                            const float* __restrict__ bias, int relu,
+                                                            uint16_t* __restrict__ out)
+{
+    constexpr int HX = kTileX + 2, HY = TY + 2, NV = halo_vox(TY), RPW = TY * kTileZ / 4;  // rows per wave
+    extern __shared__ uint4 tile4[];
+    const int Cin = in.ca + in.cb, K = 27 * Cin, Cout = 16 * NT;
+    const int tx = blockIdx.x * kTileX, ty = blockIdx.y * TY, tz = blockIdx.z * kTileZ;
+    const int groups = Cin / 8;
+    for (int t = threadIdx.x; t < NV * groups; t += kBlock) {
+        const int v = t / groups, cg = t - v * groups;
+        const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+        tile4[(v * Cin + cg * 8) / 8] = load_in8(in, D, H, W, tz + hz - 1, ty + hy - 1, tx + hx - 1, cg * 8);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, i = lane & 15;
+    f32x4 acc[RPW][NT];
+#pragma unroll
+    for (int rr = 0; rr < RPW; rr++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[rr][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < Kp; kc += 32) {
+        const int k = kc + 8 * g;
+        const int tap = k / Cin, c = k - tap * Cin;
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // offsets + 1
+        bf16x8 b[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) b[t] = as_bf16x8(ld16(Wt + (int64_t)(16 * t + i) * Kp + k));
+#pragma unroll
+        for (int rr = 0; rr < RPW; rr++) {
+            const int r = RPW * wave + rr, ry = r % TY, rz = r / TY;
+            uint4 av = make_uint4(0, 0, 0, 0);
+            if (k < K) av = tile4[((((rz + dz) * HY + (ry + dy)) * HX + (i + dx)) * Cin + c) / 8];
+            const bf16x8 a = as_bf16x8(av);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[rr][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[rr][t], 0, 0, 0);
+        }
+    }
+    // D[row][col]: lane holds rows 4g..4g+3 (x = tx + 4g + r) of column i
+#pragma unroll
+    for (int rr = 0; rr < RPW; rr++) {
+        const int rw = RPW * wave + rr, y = ty + rw % TY, z = tz + rw / TY;
+        if (y >= H || z >= D) continue;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int co = 16 * t + i;
+            const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int x = tx + 4 * g + r;
+                if (x < W) {
+                    float v = acc[rr][t][r] + bv;
+                    if (relu) v = v > 0.0f ? v : 0.0f;
+                    out[(((int64_t)z * H + y) * W + x) * Cout + co] = f2bf(v);
+                }
+            }
+        }
+    }
+}
+
 // Weight gradient partials: partial[chunk][co][k] = sum over the chunk's voxels v of
 // A[v][k] * G[v][co], A the convolution input gathered as in the forward pass, G = dL/d(pre-ReLU).
 // Workgroup: one 64-wide k tile (16 k per wave) x all Cout, over vchunk voxels in slices of 32.
@@ -524,6 +595,32 @@ int launch_conv(pct_net* net, const ConvIn& in, int level, int Kp, const uint16_
     const int64_t nvox = (int64_t)D * H * W;
     const unsigned grid = (unsigned)((nvox + 63) / 64);
     hipStream_t s = net->stream;
+    // the input halo in LDS: 16 x 4 x 2 output tiles, or 16 x 2 x 2 for wider inputs
+    const int cin = in.ca + in.cb;
+    const int ty = (size_t)halo_vox(4) * cin * sizeof(uint16_t) <= 64 * 1024 ? 4
+                   : (size_t)halo_vox(2) * cin * sizeof(uint16_t) <= 64 * 1024 ? 2 : 0;
+    if (ty > 0) {
+        const size_t lds = (size_t)halo_vox(ty) * cin * sizeof(uint16_t);
+        const dim3 tg((W + kTileX - 1) / kTileX, (H + ty - 1) / ty, (D + kTileZ - 1) / kTileZ);
+#define PCT_CONV_T(NT)                                                                                            \
+    if (ty == 4)                                                                                                  \
+        hipLaunchKernelGGL((k_conv_fwd_tiled<NT, 4>), tg, dim3(kBlock), lds, s, in, D, H, W, Kp, w, bias, relu, out); \
+    else                                                                                                          \
+        hipLaunchKernelGGL((k_conv_fwd_tiled<NT, 2>), tg, dim3(kBlock), lds, s, in, D, H, W, Kp, w, bias, relu, out);
+        switch (cout) {
+        case 16: PCT_CONV_T(1) break;
+        case 32: PCT_CONV_T(2) break;
+        case 48: PCT_CONV_T(3) break;
+        case 64: PCT_CONV_T(4) break;
+        case 96: PCT_CONV_T(6) break;
+        case 128: PCT_CONV_T(8) break;
+        case 192: PCT_CONV_T(12) break;
+        default: net->err = "unsupported channel count"; return PCT_E_ARG;
+        }
+#undef PCT_CONV_T
+        NET_HIP(net, hipGetLastError());
+        return PCT_OK;
+    }
     switch (cout) {
     case 16: hipLaunchKernelGGL(k_conv_fwd<1>, dim3(grid), dim3(kBlock), 0, s, in, D, H, W, Kp, w, bias, relu, out); break;
     case 32: hipLaunchKernelGGL(k_conv_fwd<2>, dim3(grid), dim3(kBlock), 0, s, in, D, H, W, Kp, w, bias, relu, out); break;
