@@ -112,6 +112,7 @@ struct DropPlan {
     int64_t cap_entries = 0, cap_off = 0, cap_r = 0, cap_perm = 0, cap_group_e = 0, cap_group_h = 0,
             cap_group_b = 0;
     int flags = -1;              // PCT_DROP_* flags the plan was built with
+    bool desc_all = false;       // desc holds a descriptor for every reconstruction history
     bool ready = false;
 };
 

@@ -340,6 +340,39 @@ __global__ void __launch_bounds__(kBlock) k_residual(int64_t n, const int64_t* _
     }
 }
 
+// k_residual over the DROP plan's row descriptors: every history once, in the blocks' visiting order
+// (neighbouring parallel paths share a wavefront, as in k_drop_rows).
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_residual_desc(int64_t n, const RowDesc* __restrict__ desc,
+                                                           const uint32_t* __restrict__ vox, const float* __restrict__ x,
+                                                           int min_len, double* __restrict__ r2)
+{
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (p >= n) return;  // the G lanes of a history leave together
+    const RowDesc d = desc[p];
+    const uint32_t* __restrict__ lst = vox + d.off;
+    const int cnt = (int)d.cnt;
+    double part = 0.0;
+    for (int j = lane; j < cnt; j += kUnroll * G) {
+        uint32_t a[kUnroll];
+        float xv[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) a[q] = j + q * G < cnt ? lst[j + q * G] : 0u;
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) xv[q] = j + q * G < cnt ? x[a[q]] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++)
+            if (j + q * G < cnt) part += (double)xv[q];
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
+    if (lane == 0) {
+        const double r = (double)d.wepl - d.chord * part;
+        r2[p] = cnt >= min_len ? r * r : 0.0;
+    }
+}
+
 // Running per-voxel sum in history order: FP64 (s = u_1 + u_2 + ...), FP32 (the reference's float
 // sums), or FP32 with Kahan compensation (from s = c = 0).
 template <int ACC>
@@ -911,6 +944,10 @@ int pct_drop_prepare(pct_ctx* ctx)
     }
     if ((ctx->drop_flags & PCT_DROP_COLS_SLICED) && !(ctx->drop_flags & PCT_DROP_COLS_LDS))
         PCT_TRY(build_slices(ctx, D));
+    // the groups' descriptors cover every history when the groups tile [0, n)
+    D.desc_all = !D.groups.empty() && D.groups.front().hist_begin == 0 && D.groups.back().hist_end == n;
+    for (size_t gi = 1; gi < D.groups.size(); gi++)
+        D.desc_all = D.desc_all && D.groups[gi].hist_begin == D.groups[gi - 1].hist_end;
     D.flags = ctx->drop_flags;
     D.ready = true;
     (void)nb;
@@ -1076,8 +1113,12 @@ int pct_stage_residual(pct_ctx* ctx, double* norm)
             x = D.x_b;
         }
         constexpr int G = 16;
-        hipLaunchKernelGGL(k_residual<G>, dim3(grid_for(n * G)), dim3(kBlock), 0, s, n, ctx->list_off, ctx->list_cnt,
-                           vox, ctx->chord, ctx->wepl, x, ctx->P.min_mlp_length, ctx->res_r2);
+        if (vox == D.vox_b && D.desc_all)  // the plan's descriptors cover every history
+            hipLaunchKernelGGL(k_residual_desc<G>, dim3(grid_for(n * G)), dim3(kBlock), 0, s, n, D.desc, vox, x,
+                               ctx->P.min_mlp_length, ctx->res_r2);
+        else
+            hipLaunchKernelGGL(k_residual<G>, dim3(grid_for(n * G)), dim3(kBlock), 0, s, n, ctx->list_off,
+                               ctx->list_cnt, vox, ctx->chord, ctx->wepl, x, ctx->P.min_mlp_length, ctx->res_r2);
         PCT_CHECK_HIP(ctx, hipGetLastError());
         size_t need = 0;
         PCT_CHECK_HIP(ctx, hipcub::DeviceReduce::Sum(nullptr, need, ctx->res_r2, out, n, s));
