@@ -1088,7 +1088,7 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
     return PCT_OK;
 }
 
-// ||b - A x|| over the reconstruction histories (the judge of the learned start and of the tests).
+// ||b - A x|| over the reconstruction histories: the criterion that keeps or rejects the learned start.
 // With the brick layout prepared, the lists' brick indices gather from a brick-ordered copy of x.
 int pct_stage_residual(pct_ctx* ctx, double* norm)
 {
