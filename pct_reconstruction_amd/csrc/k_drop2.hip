@@ -154,22 +154,23 @@ __global__ void k_group_long_counts(int64_t h0, int64_t n, int64_t BS, const int
     out[t] = (i - rb * BS < proc_raw[rb] && (int)c >= min_len) ? c : 0u;
 }
 
-// ((block in group, voxel), rank) pairs in history order; one wavefront per history, coalesced
+// ((block in group, voxel), rank) pairs in history order; 16 lanes per history, coalesced
 __global__ void k_group_emit(int64_t h0, int64_t n, int64_t BS, int64_t rb0, int vbits,
                              const int64_t* __restrict__ off, const uint32_t* __restrict__ vox,
                              const uint32_t* __restrict__ long_cnt, const uint32_t* __restrict__ pos,
                              const uint32_t* __restrict__ rank, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n; t += n_waves) {
+    constexpr int L = 16;  // lanes per history: four histories' loads in flight per wavefront
+    const int lane = threadIdx.x & (L - 1);
+    const int64_t n_groups = ((int64_t)gridDim.x * blockDim.x) / L;
+    for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L; t < n; t += n_groups) {
         const uint32_t c = long_cnt[t];
         if (c == 0) continue;
         const int64_t i = h0 + t;
         const uint32_t tag = (uint32_t)(i / BS - rb0) << vbits;
         const int64_t base = off[i];
         const uint32_t p = pos[t], rk = rank[i];
-        for (uint32_t j = lane; j < c; j += kWave) {
+        for (uint32_t j = lane; j < c; j += L) {
             keys[p + j] = tag | vox[base + j];
             vals[p + j] = rk;
         }
@@ -878,7 +879,7 @@ int pct_drop_prepare(pct_ctx* ctx)
                                ctx->list_cnt, P.min_mlp_length, D.lc);
             tb = ctx->cub_tmp_bytes;
             PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.lc, D.pos, (int)H, s));
-            int64_t wg = (H * kWave + kBlock - 1) / kBlock;
+            int64_t wg = (H * 16 + kBlock - 1) / kBlock;  // 16 lanes per history
             if (wg > 65536) wg = 65536;
             hipLaunchKernelGGL(k_group_emit, dim3((unsigned)wg), dim3(kBlock), 0, s, h0, H, BS, rb0, vbits, ctx->list_off,
                                lvox, D.lc, D.pos, D.rank, D.keys, D.vals);
