@@ -221,14 +221,15 @@ int pct_load_histories(pct_ctx* ctx, size_t n, const float* const t[4], const fl
     if (total > ctx->cap_loaded) {
         int64_t cap = total > 2 * ctx->cap_loaded ? total : 2 * ctx->cap_loaded;
         for (int k = 0; k < 14; k++) {
-            float* nb = nullptr;
-            PCT_TRY(pct_alloc(ctx, &nb, cap));
+            pct_tmp<float> nb;
+            PCT_TRY(pct_alloc(ctx, &nb.p, cap));
             if (ctx->raw[k] && ctx->n_loaded)
-                PCT_CHECK_HIP(ctx, hipMemcpyAsync(nb, ctx->raw[k], ctx->n_loaded * sizeof(float),
+                PCT_CHECK_HIP(ctx, hipMemcpyAsync(nb.p, ctx->raw[k], ctx->n_loaded * sizeof(float),
                                                   hipMemcpyDeviceToDevice, ctx->stream));
             PCT_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
             pct_free(&ctx->raw[k]);
-            ctx->raw[k] = nb;
+            ctx->raw[k] = nb.p;
+            nb.p = nullptr;  // owned by the context from here on
         }
         ctx->cap_loaded = cap;
     }
@@ -376,7 +377,7 @@ int pct_get_image(pct_ctx* ctx, int kind, float* out)
     if (!ctx || !out) return PCT_E_ARG;
     const int64_t NV = ctx->P.num_voxels;
     hipStream_t s = ctx->stream;
-    float* tmp = nullptr;
+    pct_tmp<float> tmp;
     const void* src = nullptr;
     int type = 0;  // 0 float, 1 u8, 2 i32
     switch (kind) {
@@ -401,15 +402,14 @@ int pct_get_image(pct_ctx* ctx, int kind, float* out)
     if (type == 0) {
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(out, src, NV * sizeof(float), hipMemcpyDeviceToHost, s));
     } else {
-        PCT_TRY(pct_alloc(ctx, &tmp, NV));
+        PCT_TRY(pct_alloc(ctx, &tmp.p, NV));
         if (type == 1)
-            hipLaunchKernelGGL(k_to_float<uint8_t>, dim3(grid_for(NV)), dim3(kBlock), 0, s, NV, (const uint8_t*)src, tmp);
+            hipLaunchKernelGGL(k_to_float<uint8_t>, dim3(grid_for(NV)), dim3(kBlock), 0, s, NV, (const uint8_t*)src, tmp.p);
         else
-            hipLaunchKernelGGL(k_to_float<int32_t>, dim3(grid_for(NV)), dim3(kBlock), 0, s, NV, (const int32_t*)src, tmp);
-        PCT_CHECK_HIP(ctx, hipMemcpyAsync(out, tmp, NV * sizeof(float), hipMemcpyDeviceToHost, s));
+            hipLaunchKernelGGL(k_to_float<int32_t>, dim3(grid_for(NV)), dim3(kBlock), 0, s, NV, (const int32_t*)src, tmp.p);
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(out, tmp.p, NV * sizeof(float), hipMemcpyDeviceToHost, s));
     }
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    pct_free(&tmp);
     return PCT_OK;
 }
 
@@ -448,12 +448,11 @@ int pct_get_bins(pct_ctx* ctx, int kind, float* out)
         return PCT_E_STATE;
     }
     if (is_int) {
-        float* tmp = nullptr;
-        PCT_TRY(pct_alloc(ctx, &tmp, NB));
-        hipLaunchKernelGGL(k_to_float<int32_t>, dim3(grid_for(NB)), dim3(kBlock), 0, s, NB, (const int32_t*)src, tmp);
-        PCT_CHECK_HIP(ctx, hipMemcpyAsync(out, tmp, NB * sizeof(float), hipMemcpyDeviceToHost, s));
+        pct_tmp<float> tmp;
+        PCT_TRY(pct_alloc(ctx, &tmp.p, NB));
+        hipLaunchKernelGGL(k_to_float<int32_t>, dim3(grid_for(NB)), dim3(kBlock), 0, s, NB, (const int32_t*)src, tmp.p);
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(out, tmp.p, NB * sizeof(float), hipMemcpyDeviceToHost, s));
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-        pct_free(&tmp);
     } else {
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(out, src, NB * sizeof(float), hipMemcpyDeviceToHost, s));
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));

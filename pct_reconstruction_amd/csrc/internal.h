@@ -245,6 +245,15 @@ void pct_free(T** p)
     if (*p) (void)hipFree(*p);
     *p = nullptr;
 }
+// A device buffer owned by one stage call: freed when the call returns, on its error paths too.
+template <class T>
+struct pct_tmp {
+    T* p = nullptr;
+    pct_tmp() = default;
+    pct_tmp(const pct_tmp&) = delete;
+    pct_tmp& operator=(const pct_tmp&) = delete;
+    ~pct_tmp() { pct_free(&p); }
+};
 
 DevGeom pct_geom(const pct_params& P);
 

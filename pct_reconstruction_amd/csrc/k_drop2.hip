@@ -726,19 +726,21 @@ int pct_drop_prepare(pct_ctx* ctx)
         const int64_t proc = P.quirk_a3_drop_floor ? (m / P.histories_per_block) * P.histories_per_block : m;
         proc_raw[rb] = proc > 0 ? (int32_t)proc : 0;
     }
-    int32_t* proc_dev = nullptr;
-    int64_t* e_dev = nullptr;
-    PCT_TRY(pct_alloc(ctx, &proc_dev, n_raw + 1));
-    PCT_TRY(pct_alloc(ctx, &e_dev, n_raw + 1));
+    pct_tmp<int32_t> proc_buf;
+    pct_tmp<int64_t> e_buf;
+    PCT_TRY(pct_alloc(ctx, &proc_buf.p, n_raw + 1));
+    PCT_TRY(pct_alloc(ctx, &e_buf.p, n_raw + 1));
+    const int32_t* proc_dev = proc_buf.p;
+    int64_t* e_dev = e_buf.p;
     std::vector<int64_t> e_raw(n_raw > 0 ? n_raw : 1, 0);
     if (n_raw > 0) {
-        PCT_CHECK_HIP(ctx, hipMemcpyAsync(proc_dev, proc_raw.data(), n_raw * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(proc_buf.p, proc_raw.data(), n_raw * sizeof(int32_t), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_block_entries, dim3((unsigned)n_raw), dim3(256), 0, s, BS, proc_dev, ctx->list_cnt,
                            P.min_mlp_length, e_dev);
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(e_raw.data(), e_dev, n_raw * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
     }
-    pct_free(&e_dev);
+    pct_free(&e_buf.p);
     int64_t total = 0, max_proc = 0;
     for (int64_t rb = 0; rb < n_raw; rb++) {
         const int64_t start = rb * BS;
@@ -747,7 +749,6 @@ int pct_drop_prepare(pct_ctx* ctx)
         if (proc <= 0) continue;
         const int64_t e = e_raw[rb];
         if (e >= (1ll << 30)) {  // a group holds at least one whole block
-            pct_free(&proc_dev);
             ctx->err = "DROP block with 2^30 or more list entries";
             return PCT_E_LIMIT;
         }
@@ -895,7 +896,6 @@ int pct_drop_prepare(pct_ctx* ctx)
             PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
             const int64_t nr = (int64_t)n_runs_h[gi];
             if (nr < 1 || nr > E) {
-                pct_free(&proc_dev);
                 ctx->err = "DROP transposition: run count out of range in a group";
                 return PCT_E_LIMIT;
             }
@@ -913,14 +913,13 @@ int pct_drop_prepare(pct_ctx* ctx)
                                           hipMemcpyDeviceToHost, s));
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));  // run_start and n_runs are reused by the next group
         if ((int64_t)n_runs_h[gi] > E) {
-            pct_free(&proc_dev);
             ctx->err = "DROP transposition: more runs than entries in a group";
             return PCT_E_LIMIT;
         }
         run_off += (int64_t)n_runs_h[gi];
     }
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    pct_free(&proc_dev);
+    pct_free(&proc_buf.p);
     // per block: its runs [run_start, next run_start) within the group
     for (auto& bk : D.blocks) {
         const DropPlan::Group& g = D.groups[bk.group];

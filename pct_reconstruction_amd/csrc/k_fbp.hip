@@ -186,9 +186,10 @@ int pct_stage_fbp(pct_ctx* ctx)
         cs[a] = std::cos(ba);
         cs[P.angular_bins + a] = std::sin(ba);
     }
-    double* d_cs = nullptr;
-    PCT_TRY(pct_alloc(ctx, &d_cs, cs.size()));
-    PCT_CHECK_HIP(ctx, hipMemcpyAsync(d_cs, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    pct_tmp<double> cs_buf;
+    PCT_TRY(pct_alloc(ctx, &cs_buf.p, cs.size()));
+    const double* d_cs = cs_buf.p;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(cs_buf.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, s));
     BPArgs a;
     a.T = T;
     a.V = P.v_bins;
@@ -211,7 +212,7 @@ int pct_stage_fbp(pct_ctx* ctx)
                        ctx->fbp);
     PCT_CHECK_HIP(ctx, hipGetLastError());
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    pct_free(&d_cs);
+    pct_free(&cs_buf.p);
     PCT_TRY(pct_median_filter_fbp(ctx));  // FBP(), DS.cu:6150-6166 (x0 only)
     if (P.hull_type == PCT_HULL_FBP) PCT_TRY(pct_fbp_hull(ctx));
     ctx->stage = ST_FBP;

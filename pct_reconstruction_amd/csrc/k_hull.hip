@@ -322,13 +322,14 @@ int pct_stage_hull_finish(pct_ctx* ctx)
     }
     if (P.sm_on) {
         PCT_TRY(pct_alloc(ctx, &ctx->sm_hull, NV));
-        int32_t* thr = nullptr;
-        PCT_TRY(pct_alloc(ctx, &thr, P.slices));
+        pct_tmp<int32_t> thr_buf;
+        PCT_TRY(pct_alloc(ctx, &thr_buf.p, P.slices));
+        int32_t* thr = thr_buf.p;
         hipLaunchKernelGGL(k_sm_threshold, dim3(P.slices), dim3(kBlock), 0, s, g, ctx->sm_counts, thr);
         hipLaunchKernelGGL(k_sm_edge, dim3(grid_for(NV)), dim3(kBlock), 0, s, g, ctx->sm_counts, thr, ctx->sm_hull,
                            P.sm_scale_threshold);
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-        pct_free(&thr);
+        pct_free(&thr_buf.p);
     }
     PCT_CHECK_HIP(ctx, hipGetLastError());
     ctx->stage = ST_HULLS;

@@ -399,16 +399,17 @@ int pct_stage_intersect_bin_carve(pct_ctx* ctx)
             cs[a] = std::cos(angle);
             cs[P.angular_bins + a] = std::sin(angle);
         }
-        double* d_cs = nullptr;
-        PCT_TRY(pct_alloc(ctx, &d_cs, cs.size()));
-        PCT_CHECK_HIP(ctx, hipMemcpyAsync(d_cs, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        pct_tmp<double> cs_buf;
+        PCT_TRY(pct_alloc(ctx, &cs_buf.p, cs.size()));
+        const double* d_cs = cs_buf.p;
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(cs_buf.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_binning, dim3(g), dim3(kBlock), 0, s, n, o, missed, ctx->bin_num, ctx->in_range, d_cs,
                            d_cs + P.angular_bins, P.pi, P.angular_bin_size, P.t_bin_size, P.v_bin_size, P.t_bins,
                            P.v_bins, P.angular_bins);
         hipLaunchKernelGGL(k_not, dim3(g), dim3(kBlock), 0, s, n, missed, ctx->flag);
         PCT_CHECK_HIP(ctx, hipGetLastError());
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
-        pct_free(&d_cs);
+        pct_free(&cs_buf.p);
     }
     // keep histories that traversed the reconstruction volume (DS.cu:5429-5449)
     int64_t kept = 0;
