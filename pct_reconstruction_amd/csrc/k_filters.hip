@@ -12,7 +12,8 @@
 //
 // The median runs per 16 x 16 tile of a slice with the (16 + 2r)^2 input patch in LDS; each lane
 // selects its median by rank: the window element v with #(w < v) <= mid < #(w <= v) is exactly the
-// element a sort puts at position mid.
+// element a sort puts at position mid.  For r <= 5 each lane holds its window in registers
+// (k_median2d_fixed); larger radii compare from LDS.
 #include "internal.h"
 
 namespace {
@@ -56,6 +57,48 @@ __global__ void __launch_bounds__(kTile* kTile) k_median2d(int C, int R, int r, 
     out[(int64_t)z * C * R + (int64_t)y * C + x] = res;
 }
 
+// k_median2d for a radius known at compile time (RR <= 5): each lane holds its (2RR+1)^2 window in
+// registers and ranks the candidates, taken from LDS in the same order, against it.  The first
+// candidate with lt <= mid < le is the one k_median2d returns, so the images are identical.
+template <int RR>
+__global__ void __launch_bounds__(kTile* kTile) k_median2d_fixed(int C, int R, const float* __restrict__ in,
+                                                                 float* __restrict__ out)
+{
+    constexpr int W = 2 * RR + 1, N = W * W, MID = N / 2, P = kTile + 2 * RR;
+    __shared__ float patch[P * P];
+    const int z = blockIdx.z;
+    const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+    const float* slice = in + (int64_t)z * C * R;
+    for (int t = threadIdx.y * kTile + threadIdx.x; t < P * P; t += kTile * kTile) {
+        const int px = x0 - RR + t % P, py = y0 - RR + t / P;
+        patch[t] = (px >= 0 && px < C && py >= 0 && py < R) ? slice[(int64_t)py * C + px] : 0.0f;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= C || y >= R) return;
+    float res = 0.0f;
+    if (x >= RR && x < C - RR && y >= RR && y < R - RR) {
+        const float* wp = patch + threadIdx.y * P + threadIdx.x;
+        float win[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) win[j] = wp[(j / W) * P + j % W];
+        for (int i = 0; i < N; i++) {
+            const float v = wp[(i / W) * P + i % W];
+            int lt = 0, le = 0;
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                lt += win[j] < v;
+                le += win[j] <= v;
+            }
+            if (lt <= MID && MID < le) {
+                res = v;
+                break;
+            }
+        }
+    }
+    out[(int64_t)z * C * R + (int64_t)y * C + x] = res;
+}
+
 __global__ void k_avg_hull(int C, int R, int64_t nv, int r, double thr, const uint8_t* __restrict__ in,
                            uint8_t* __restrict__ out)
 {
@@ -89,7 +132,15 @@ int pct_median_filter_fbp(pct_ctx* ctx)
     }
     PCT_TRY(pct_alloc(ctx, &ctx->fbp_x0, P.num_voxels));
     dim3 grid((P.columns + kTile - 1) / kTile, (P.rows + kTile - 1) / kTile, P.slices), block(kTile, kTile);
-    hipLaunchKernelGGL(k_median2d, grid, block, 0, ctx->stream, P.columns, P.rows, r, ctx->fbp, ctx->fbp_x0);
+    switch (r) {
+    case 1: hipLaunchKernelGGL(k_median2d_fixed<1>, grid, block, 0, ctx->stream, P.columns, P.rows, ctx->fbp, ctx->fbp_x0); break;
+    case 2: hipLaunchKernelGGL(k_median2d_fixed<2>, grid, block, 0, ctx->stream, P.columns, P.rows, ctx->fbp, ctx->fbp_x0); break;
+    case 3: hipLaunchKernelGGL(k_median2d_fixed<3>, grid, block, 0, ctx->stream, P.columns, P.rows, ctx->fbp, ctx->fbp_x0); break;
+    case 4: hipLaunchKernelGGL(k_median2d_fixed<4>, grid, block, 0, ctx->stream, P.columns, P.rows, ctx->fbp, ctx->fbp_x0); break;
+    case 5: hipLaunchKernelGGL(k_median2d_fixed<5>, grid, block, 0, ctx->stream, P.columns, P.rows, ctx->fbp, ctx->fbp_x0); break;
+    default:
+        hipLaunchKernelGGL(k_median2d, grid, block, 0, ctx->stream, P.columns, P.rows, r, ctx->fbp, ctx->fbp_x0);
+    }
     PCT_CHECK_HIP(ctx, hipGetLastError());
     return PCT_OK;
 }

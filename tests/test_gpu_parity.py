@@ -137,6 +137,23 @@ def test_parity_head_with_fbp_median_and_hull_average_filters(oracle):
     _run_and_compare(P, data, oracle, "head-filters")
 
 
+@pytest.mark.parametrize("radius", [1, 3, 5, 6])
+def test_fbp_median_filter_radii(oracle, radius):
+    """The FBP median filter at radius 1-5 (windows held in registers) and 6 (windows compared from
+    LDS), against the oracle's filter of the library's own FBP image; radius 3 is the reference's
+    FBP_MED_FILTER_RADIUS."""
+    from pct_reconstruction_amd import Reconstruction
+    P, data = small_head(n=50000, seed=11, median_filter_fbp=1, fbp_med_filter_radius=radius)
+    with Reconstruction(P) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.preprocessing()
+        fbp = r.image("fbp")
+        got = r.image("fbp_median").reshape(-1)
+    want = oracle.median_filter_2d(P, fbp, radius).reshape(-1)
+    assert np.count_nonzero(want) > 0
+    assert _mismatch(got, want) == 0
+
+
 def test_drop_transposition_groups_do_not_change_bits(monkeypatch):
     """Blocks are transposed in groups of up to 2^30 entries; forcing many small groups (one to a few
     blocks each, including the quirk-A3 block tails) must give the same image bit for bit."""
