@@ -71,6 +71,21 @@ __device__ __forceinline__ float subgradient(const float* __restrict__ x, int64_
     return g;
 }
 
+// Column and row of voxel w (w % C, (w / C) % R), and the step to voxel w + kThreads: one
+// division per lane instead of two 64-bit divisions per voxel.
+struct ColRow {
+    int col, row;
+    __device__ ColRow(int64_t w, int C, int R) : col((int)(w % C)), row((int)((w / C) % R)) {}
+    __device__ void advance(int by, int C, int R)
+    {
+        col += by;
+        while (col >= C) {
+            col -= C;
+            if (++row == R) row = 0;
+        }
+    }
+};
+
 // One term of calculate_total_variation (:10178): sqrt(powf(dy, 2) + powf(dx, 2)) in float.
 __device__ __forceinline__ float tv_term(float c, float right, float below)
 {
@@ -87,10 +102,11 @@ __global__ void __launch_bounds__(kThreads) k_tvs_grad(int C, int R, int64_t nv,
     __shared__ double a[kThreads], b[kThreads];
     double sn = 0.0, st = 0.0;
     const int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x;
-    for (int i = 0; i < 4; i++) {
+    ColRow cr(base, C, R);
+    for (int i = 0; i < 4; i++, cr.advance(kThreads, C, R)) {
         const int64_t w = base + (int64_t)kThreads * i;
         if (w >= nv) break;
-        const int col = (int)(w % C), row = (int)((w / C) % R);
+        const int col = cr.col, row = cr.row;
         if (GRAD) {
             const float g = subgradient(x, w, col, row, C, R);
             G[w] = g;
@@ -129,10 +145,11 @@ __global__ void __launch_bounds__(kThreads) k_tvs_candidate_tv(int C, int R, int
     __shared__ double a[kThreads];
     double st = 0.0;
     const int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x;
-    for (int i = 0; i < 4; i++) {
+    ColRow cr(base, C, R);
+    for (int i = 0; i < 4; i++, cr.advance(kThreads, C, R)) {
         const int64_t w = base + (int64_t)kThreads * i;
         if (w >= nv) break;
-        const int col = (int)(w % C), row = (int)((w / C) % R);
+        const int col = cr.col, row = cr.row;
         float t = 0.0f;
         if (col < C - 1 && row < R - 1)
             t = tv_term(perturbed(x, G, hull, w, norm, beta), perturbed(x, G, hull, w + 1, norm, beta),
