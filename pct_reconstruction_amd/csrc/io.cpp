@@ -10,10 +10,13 @@
 // Images: array_2_disk text layout (DS.cu:10743-10787) and write_PNG's .dat (DS.cu:9939-9973), plus the PNG
 // that write_PNG makes from the .dat with ImageMagick (DS.cu:9969-9971).
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pct.h"
@@ -102,11 +105,44 @@ extern "C" int pct_write_image_text(const char* path, const float* image, int co
     if (!path || !image || columns <= 0 || rows <= 0 || slices <= 0) return PCT_E_ARG;
     std::ofstream out(path);
     if (!out) return PCT_E_IO;
-    for (int z = 0; z < slices; z++)
-        for (int y = 0; y < rows; y++) {
-            for (int x = 0; x < columns; x++) out << image[x + (int64_t)y * columns + (int64_t)z * columns * rows] << " ";
-            out << std::endl;
+    // Each slice is formatted by its own default-format stream (the format of `out`: precision 6,
+    // the global locale), up to 16 slices at a time on separate threads, and the slices are written
+    // in order.  '\n' writes the byte std::endl writes, without a flush per line.
+    const int64_t plane = (int64_t)columns * rows;
+    std::atomic<bool> failed{false};
+    auto format_slice = [&](int z, std::string& dst) {
+        try {
+            std::ostringstream ss;
+            const float* p = image + (int64_t)z * plane;
+            for (int y = 0; y < rows; y++) {
+                for (int x = 0; x < columns; x++) ss << p[x + (int64_t)y * columns] << " ";
+                ss << '\n';
+            }
+            dst = ss.str();
+        } catch (...) {
+            failed = true;
         }
+    };
+    unsigned hw = std::thread::hardware_concurrency();
+    const int width = (int)std::min(16u, hw > 0 ? hw : 1u);
+    std::vector<std::string> text(width);
+    for (int z0 = 0; z0 < slices; z0 += width) {
+        const int nb = std::min(width, slices - z0);
+        std::vector<std::thread> pool;
+        for (int k = 1; k < nb; k++) {
+            try {
+                pool.emplace_back(format_slice, z0 + k, std::ref(text[k]));
+            } catch (const std::system_error&) {
+                format_slice(z0 + k, text[k]);  // no thread available: format here
+            }
+        }
+        format_slice(z0, text[0]);
+        for (auto& t : pool) t.join();
+        if (failed) return PCT_E_NOMEM;
+        for (int k = 0; k < nb; k++) out.write(text[k].data(), (std::streamsize)text[k].size());
+        if (!out) return PCT_E_IO;
+    }
+    out.flush();
     return out ? PCT_OK : PCT_E_IO;
 }
 

@@ -79,6 +79,18 @@ def test_image_writers(tmp_path):
     assert np.array_equal(np.fromfile(tmp_path / "x.dat", np.float32), (img / 2).reshape(-1))
 
 
+def test_text_writer_bytes_match_the_default_stream_format(tmp_path):
+    """array_2_disk writes `value << " "` per voxel and std::endl per row with the default stream
+    format, which is C's %g (precision 6).  20 slices: more than the writer formats at once."""
+    rng = np.random.default_rng(12)
+    img = (rng.standard_normal((20, 7, 9)) * np.exp(rng.uniform(-25, 25, (20, 7, 9)))).astype(np.float32)
+    img[0, 0, :6] = [0.0, -0.0, np.inf, -np.inf, 1e-45, 3.4e38]
+    io.write_image_text(tmp_path / "x.txt", img)
+    want = "".join("".join(f"{float(v):g} " for v in row) + "\n" for row in img.reshape(-1, img.shape[2]))
+    assert want.startswith("0 -0 inf -inf 1.4013e-45 3.4e+38 ")
+    assert (tmp_path / "x.txt").read_bytes() == want.encode()
+
+
 def test_png_writer_decodes_to_the_quantized_image(tmp_path):
     from PIL import Image
     rng = np.random.default_rng(4)
