@@ -317,7 +317,8 @@ int pct_read_projection(const char* path, int64_t* n, float* header_angle_deg, f
 int pct_write_projection(const char* path, int64_t n, float angle_deg, float energy_mev, const float* const t[4],
                          const float* const v[4], const float* const u[4], const float* wepl, const float* angles);
 /* Text image in the array_2_disk layout (DS.cu:10743): COLUMNS values + ' ' per line, ROWS*SLICES lines,
- * default ostream formatting (6 significant digits). */
+ * default ostream formatting (6 significant digits).  Slices are formatted on up to 16 host threads
+ * and written in order; PCT_E_NOMEM if formatting runs out of memory, PCT_E_IO on a write error. */
 int pct_write_image_text(const char* path, const float* image, int columns, int rows, int slices);
 /* Binary image of write_PNG (DS.cu:9939): float32 value/2 in x-fastest order. */
 int pct_write_image_dat(const char* path, const float* image, int columns, int rows, int slices);
