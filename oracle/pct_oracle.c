@@ -861,7 +861,7 @@ double o_chord_length(const pct_params* p, double abs_angle_t, double abs_angle_
  * (or -1 past cap) and the chord.  Voxels whose linear index is >= NUM_VOXELS are not listed (the
  * reference reads and writes out of bounds there). */
 static int64_t o_mlp_path(const pct_params* p, const o_tables* tb, const float* const h[10], int64_t i, uint32_t first,
-                          uint32_t* path, int64_t cap, double* chord)
+                          uint32_t* path, int64_t cap, double* chord, double* rec, int64_t rec_cap, int64_t* n_rec)
 {
     const double PI = p->pi, STEP = p->mlp_u_step;
     const int C = p->columns, R = p->rows;
@@ -930,6 +930,11 @@ static int64_t o_mlp_path(const pct_params* p, const o_tables* tb, const float* 
         int vy = o_calc_voxel(p->y_zero_coordinate, y_1, p->voxel_height);
         int vz = o_calc_voxel(p->z_zero_coordinate, v_1, p->voxel_thickness);
         int64_t voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
+        if (rec && *n_rec < rec_cap) {   /* per-step record for o_mlp_steps */
+            double* r = rec + 6 * *n_rec;
+            r[0] = u_shifted; r[1] = t_1; r[2] = v_1; r[3] = x_1; r[4] = y_1; r[5] = (double)voxel;
+        }
+        if (rec) *n_rec += 1;
         if (voxel != (int64_t)last && voxel < NV) {
             if (n >= cap) return -1;
             path[n++] = (uint32_t)voxel;
@@ -965,7 +970,7 @@ int64_t o_mlp_lists(const pct_params* p, const o_tables* t, int64_t n, const flo
         int64_t used = 0;
         for (int64_t i = a; i < b && buf; i++) {
             double L;
-            int64_t k = o_mlp_path(p, t, h, i, first_voxel[i], buf + used, cap - used, &L);
+            int64_t k = o_mlp_path(p, t, h, i, first_voxel[i], buf + used, cap - used, &L, NULL, 0, NULL);
             if (k < 0) { failed = 1; break; }
             used += k;
             offsets[i + 1] = k;   /* count for now */
@@ -991,6 +996,20 @@ int64_t o_mlp_lists(const pct_params* p, const o_tables* t, int64_t n, const flo
     free(cbuf);
     free(csize);
     return total;
+}
+
+int64_t o_mlp_steps(const pct_params* p, const o_tables* t, const float* hist10, uint32_t first, double* rec,
+                    int64_t rec_cap, uint32_t* path, int64_t path_cap, int64_t* path_len, double* chord)
+{
+    float one[10];
+    const float* h[10];
+    for (int k = 0; k < 10; k++) { one[k] = hist10[k]; h[k] = &one[k]; }
+    int64_t n_rec = 0;
+    double L = 0.0;
+    int64_t n = o_mlp_path(p, t, h, 0, first, path, path_cap, &L, rec, rec_cap, &n_rec);
+    if (path_len) *path_len = n;
+    if (chord) *chord = L;
+    return n_rec;
 }
 
 /* ------------------------------------------------------------------ (e) DROP */

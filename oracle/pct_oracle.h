@@ -97,6 +97,13 @@ double o_chord_length(const pct_params* p, double abs_angle_t, double abs_angle_
 int64_t o_mlp_lists(const pct_params* p, const o_tables* t, int64_t n, const float* const h[10],
                     const uint32_t* first_voxel, int64_t* offsets, uint32_t* voxels, int64_t capacity, double* chord);
 
+/* The MLP loop of o_mlp_lists for one history (hist10 = the 10 geometry values of one history, in the
+ * h[] order), recording every u-step: rec[6 k + 0..5] = u (the rotated depth u_shifted), t_1, v_1, x_1,
+ * y_1 and the voxel index of step k.  Returns the number of steps (at most rec_cap are recorded); the
+ * voxel list goes to path (*path_len entries, -1 past path_cap) and the chord to *chord. */
+int64_t o_mlp_steps(const pct_params* p, const o_tables* t, const float* hist10, uint32_t first, double* rec,
+                    int64_t rec_cap, uint32_t* path, int64_t path_cap, int64_t* path_len, double* chord);
+
 /* One DROP iteration over precomputed lists (DROP_full_tx_iteration :9537, calculate_x_update_GPU
  * :9321, DROP_block_update_GPU :9522, image_update_GPU :9398) in history order.
  * accum = PCT_ACCUM_FP32: x_update in float (reference); PCT_ACCUM_FP64: x_update in double.

@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
         "o_endpoints": ([P(Params), vp, i64, f10, vp], None),
         "o_chord_length": ([P(Params), C.c_double, C.c_double], C.c_double),
         "o_mlp_lists": ([P(Params), vp, i64, f10, vp, vp, vp, i64, vp], i64),
+        "o_mlp_steps": ([P(Params), vp, vp, C.c_uint32, vp, i64, vp, i64, vp, vp], i64),
         "o_drop_iteration": ([P(Params), i64, vp, vp, vp, vp, vp, C.c_int, C.c_int], None),
         "o_drop_iteration_fp64": ([P(Params), i64, vp, vp, vp, vp, vp], None),
         "o_residual": ([i64, vp, vp, vp, vp, vp, C.c_int], C.c_double),
@@ -243,6 +244,22 @@ def mlp_lists(P, tables: MlpTables, h, first):
     if total < 0:
         raise RuntimeError("oracle MLP capacity")
     return off, vox[:total].copy(), chord
+
+
+def mlp_steps(P, tables: MlpTables, hist10, first):
+    """Every u-step of the MLP loop for one history: returns (steps, path, chord) where steps is a
+    (k, 6) array of u, t_1, v_1, x_1, y_1, voxel."""
+    h = np.ascontiguousarray(hist10, np.float32).reshape(10)
+    cap = 8 * (int(2 * P.recon_cyl_radius / P.mlp_u_step) + 8)
+    rec = np.zeros((cap, 6), np.float64)
+    path = np.zeros(cap, np.uint32)
+    plen = C.c_int64()
+    chord = C.c_double()
+    k = lib().o_mlp_steps(C.byref(P), C.byref(tables.t), _p(h), int(first), _p(rec), cap, _p(path), cap,
+                          C.byref(plen), C.byref(chord))
+    if k > cap or plen.value < 0:
+        raise RuntimeError("oracle MLP step capacity")
+    return rec[:k].copy(), path[:plen.value].copy(), chord.value
 
 
 def drop_iteration(P, off, vox, chord, wepl, x, accum=1, nthreads=1):
