@@ -67,19 +67,6 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
-MICROBENCH = ROOT / "pct_reconstruction_amd" / "microbench"
-
-
-def build_microbench() -> None:
-    """The gather-rate microbenchmark (a standalone executable beside its source)."""
-    src = MICROBENCH / "gather_rate.hip"
-    out = MICROBENCH / "gather_rate"
-    tmp = out.with_suffix(".tmp")
-    _run([_hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-          "-packed-fp32-ops", "-o", str(tmp), str(src)])
-    os.replace(tmp, out)
-
-
 def build_oracle() -> Path:
     ORACLE_LIB.parent.mkdir(exist_ok=True)
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
@@ -93,7 +80,6 @@ def build_oracle() -> Path:
 def build_all(verbose: bool = False) -> None:
     build_library(verbose)
     build_oracle()
-    build_microbench()
 
 
 if __name__ == "__main__":

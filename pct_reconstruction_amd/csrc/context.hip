@@ -19,11 +19,7 @@
 namespace {
 
 constexpr int kBlock = 256;
-inline unsigned grid_for(int64_t n, int block = kBlock)
-{
-    int64_t g = (n + block - 1) / block;
-    return (unsigned)(g < 1 ? 1 : g);
-}
+inline unsigned grid_for(int64_t n, int block = kBlock) { return pct_grid_for(n, block); }
 
 enum TimerSlot { TM_INTERSECT = 0, TM_HULLS, TM_CUTS, TM_FBP, TM_HULL_SEL, TM_X0, TM_ENDPOINTS, TM_LISTS, TM_DROP };
 const char* kTimerName[] = {"intersect_bin_carve", "hull_finish", "stats_and_cuts", "fbp", "select_hull",

@@ -8,6 +8,7 @@
 
 #include "../../include/pct.h"
 #include "devmath.h"
+#include "launch.h"
 
 #define PCT_CHECK_HIP(ctx, expr)                                                        \
     do {                                                                                \

@@ -265,12 +265,33 @@ __global__ void __launch_bounds__(kWave) k_slice_fill(int64_t NS, int64_t chunk,
     }
 }
 
-// per history: u_i (DS.cu:9321-9380 with the list of find_MLP_path_GPU_tabulated).
-// G lanes per history (G | 64), histories taken in rank order: lane l sums entries l, l+G, l+2G, ...
-// with kUnroll list loads and then kUnroll gathers in flight, then the G partial sums are combined
-// with a butterfly.
 constexpr int kUnroll = 8;
 
+// a_i.x / L_i of one history: the sum of x over its list with the row kernel's lanes and loads
+// (G lanes, kUnroll list loads and then kUnroll gathers in flight per lane, lane l summing entries
+// l, l + G, l + 2G, ... in order, then a butterfly over the G partial sums).
+template <int G>
+__device__ inline double row_sum(const uint32_t* __restrict__ lst, int cnt, const float* __restrict__ x, int lane)
+{
+    double part = 0.0;
+    for (int j = lane; j < cnt; j += kUnroll * G) {
+        uint32_t a[kUnroll];
+        float xv[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) a[q] = j + q * G < cnt ? lst[j + q * G] : 0u;
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++) xv[q] = j + q * G < cnt ? x[a[q]] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < kUnroll; q++)
+            if (j + q * G < cnt) part += (double)xv[q];
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
+    return part;
+}
+
+// per history: u_i (DS.cu:9321-9380 with the list of find_MLP_path_GPU_tabulated).
+// G lanes per history (G | 64), histories taken in rank order.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_drop_rows(int64_t start, int64_t proc, uint32_t per,
                                                        const RowDesc* __restrict__ desc,
@@ -282,22 +303,8 @@ __global__ void __launch_bounds__(kBlock) k_drop_rows(int64_t start, int64_t pro
     const int64_t k = (int64_t)slab_index(per) * (kBlock / G) + threadIdx.x / G;
     if (k >= proc) return;  // the G lanes of a history leave together
     const RowDesc d = desc[start + k];
-    const uint32_t* __restrict__ lst = vox + d.off;
     const int cnt = (int)d.cnt;
-    double part = 0.0;
-    for (int j = lane; j < cnt; j += kUnroll * G) {
-        uint32_t a[kUnroll];
-        float xv[kUnroll];
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++) a[q] = j + q * G < cnt ? lst[j + q * G] : 0u;
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++) xv[q] = j + q * G < cnt ? x[a[q]] : 0.0f;
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++)
-            if (j + q * G < cnt) part += (double)xv[q];
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
+    const double part = row_sum<G>(vox + d.off, cnt, x, lane);
     if (lane == 0) {
         const double L = d.chord;
         const double ax = part * L;
@@ -306,9 +313,8 @@ __global__ void __launch_bounds__(kBlock) k_drop_rows(int64_t start, int64_t pro
     }
 }
 
-// per history: (b_i - L_i sum_j x[v_ij])^2 for the data residual (0 below MIN_MLP_LENGTH), with the
-// row kernel's lanes and loads: G lanes per history, kUnroll list loads and then kUnroll gathers in
-// flight per lane, a butterfly over the G partial sums.
+// per history: (b_i - L_i sum_j x[v_ij])^2 for the data residual (0 below MIN_MLP_LENGTH), G lanes
+// per history as in the row kernel.  Grid-stride over histories: any history count fits one launch.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_residual(int64_t n, const int64_t* __restrict__ off,
                                                       const uint32_t* __restrict__ cnt_arr,
@@ -317,60 +323,34 @@ __global__ void __launch_bounds__(kBlock) k_residual(int64_t n, const int64_t* _
                                                       int min_len, double* __restrict__ r2)
 {
     const int lane = threadIdx.x & (G - 1);
-    const int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    if (p >= n) return;  // the G lanes of a history leave together
-    const uint32_t* __restrict__ lst = vox + off[p];
-    const int cnt = (int)cnt_arr[p];
-    double part = 0.0;
-    for (int j = lane; j < cnt; j += kUnroll * G) {
-        uint32_t a[kUnroll];
-        float xv[kUnroll];
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++) a[q] = j + q * G < cnt ? lst[j + q * G] : 0u;
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++) xv[q] = j + q * G < cnt ? x[a[q]] : 0.0f;
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++)
-            if (j + q * G < cnt) part += (double)xv[q];
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
-    if (lane == 0) {
-        const double r = (double)wepl[p] - chord[p] * part;
-        r2[p] = cnt >= min_len ? r * r : 0.0;
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / G);
+    for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; p < n; p += stride) {
+        const int cnt = (int)cnt_arr[p];   // the G lanes of a history loop together
+        const double part = row_sum<G>(vox + off[p], cnt, x, lane);
+        if (lane == 0) {
+            const double r = (double)wepl[p] - chord[p] * part;
+            r2[p] = cnt >= min_len ? r * r : 0.0;
+        }
     }
 }
 
 // k_residual over the DROP plan's row descriptors: every history once, in the blocks' visiting order
-// (neighbouring parallel paths share a wavefront, as in k_drop_rows).
+// (neighbouring parallel paths share a wavefront, as in k_drop_rows).  Grid-stride over histories.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_residual_desc(int64_t n, const RowDesc* __restrict__ desc,
                                                            const uint32_t* __restrict__ vox, const float* __restrict__ x,
                                                            int min_len, double* __restrict__ r2)
 {
     const int lane = threadIdx.x & (G - 1);
-    const int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    if (p >= n) return;  // the G lanes of a history leave together
-    const RowDesc d = desc[p];
-    const uint32_t* __restrict__ lst = vox + d.off;
-    const int cnt = (int)d.cnt;
-    double part = 0.0;
-    for (int j = lane; j < cnt; j += kUnroll * G) {
-        uint32_t a[kUnroll];
-        float xv[kUnroll];
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++) a[q] = j + q * G < cnt ? lst[j + q * G] : 0u;
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++) xv[q] = j + q * G < cnt ? x[a[q]] : 0.0f;
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++)
-            if (j + q * G < cnt) part += (double)xv[q];
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
-    if (lane == 0) {
-        const double r = (double)d.wepl - d.chord * part;
-        r2[p] = cnt >= min_len ? r * r : 0.0;
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / G);
+    for (int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; p < n; p += stride) {
+        const RowDesc d = desc[p];
+        const int cnt = (int)d.cnt;
+        const double part = row_sum<G>(vox + d.off, cnt, x, lane);
+        if (lane == 0) {
+            const double r = (double)d.wepl - d.chord * part;
+            r2[p] = cnt >= min_len ? r * r : 0.0;
+        }
     }
 }
 
@@ -562,11 +542,7 @@ __global__ void __launch_bounds__(kWave) k_drop_cols_lds(uint32_t U, const uint3
     if (ok && e > b) x[v] = xv + a.mean(e - b);
 }
 
-inline unsigned grid_for(int64_t n, int block = kBlock)
-{
-    int64_t g = (n + block - 1) / block;
-    return (unsigned)(g < 1 ? 1 : g);
-}
+inline unsigned grid_for(int64_t n, int block = kBlock) { return pct_grid_for(n, block); }
 
 // grid for n_logical workgroups; with XCD slabs the grid is rounded up to a multiple of 8 and
 // `per` logical workgroups go to each XCD (the extra ones find no work and return).
@@ -829,7 +805,7 @@ int pct_drop_prepare(pct_ctx* ctx)
         }
         const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
         if (!D.vox_b_ready && ctx->list_cap > 0)  // lists built without the brick index
-            hipLaunchKernelGGL(k_bricks_list, dim3(std::min(grid_for(ctx->n_recon * 16), 1u << 20)), dim3(kBlock), 0,
+            hipLaunchKernelGGL(k_bricks_list, dim3(pct_grid_capped(ctx->n_recon * 16, kBlock, 1u << 20)), dim3(kBlock), 0,
                                s, ctx->n_recon, ctx->list_off, ctx->list_cnt, ctx->list_vox, D.vox_b, bg,
                                (uint32_t)P.num_voxels);
         D.vox_b_ready = true;
@@ -1113,11 +1089,12 @@ int pct_stage_residual(pct_ctx* ctx, double* norm)
             x = D.x_b;
         }
         constexpr int G = 16;
+        const unsigned rg = pct_grid_capped(n * G, kBlock, 1u << 20);  // grid-stride kernels
         if (vox == D.vox_b && D.desc_all)  // the plan's descriptors cover every history
-            hipLaunchKernelGGL(k_residual_desc<G>, dim3(grid_for(n * G)), dim3(kBlock), 0, s, n, D.desc, vox, x,
+            hipLaunchKernelGGL(k_residual_desc<G>, dim3(rg), dim3(kBlock), 0, s, n, D.desc, vox, x,
                                ctx->P.min_mlp_length, ctx->res_r2);
         else
-            hipLaunchKernelGGL(k_residual<G>, dim3(grid_for(n * G)), dim3(kBlock), 0, s, n, ctx->list_off,
+            hipLaunchKernelGGL(k_residual<G>, dim3(rg), dim3(kBlock), 0, s, n, ctx->list_off,
                                ctx->list_cnt, vox, ctx->chord, ctx->wepl, x, ctx->P.min_mlp_length, ctx->res_r2);
         PCT_CHECK_HIP(ctx, hipGetLastError());
         size_t need = 0;

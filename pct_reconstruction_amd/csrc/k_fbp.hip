@@ -9,11 +9,7 @@
 namespace {
 
 constexpr int kBlock = 256;
-inline unsigned grid_for(int64_t n, int block = kBlock)
-{
-    int64_t g = (n + block - 1) / block;
-    return (unsigned)(g < 1 ? 1 : g);
-}
+inline unsigned grid_for(int64_t n, int block = kBlock) { return pct_grid_for(n, block); }
 
 // construct_sinogram_GPU, DS.cu:6073-6079
 __global__ void k_sino_div(int NB, const int32_t* counts, float* sino)

@@ -10,11 +10,7 @@
 namespace {
 
 constexpr int kBlock = 256;
-inline unsigned grid_for(int64_t n, int block = kBlock)
-{
-    int64_t g = (n + block - 1) / block;
-    return (unsigned)(g < 1 ? 1 : g);
-}
+inline unsigned grid_for(int64_t n, int block = kBlock) { return pct_grid_for(n, block); }
 
 // initialize_hull_GPU, DS.cu:6659-6669
 __device__ __forceinline__ int inside_cylinder(const DevGeom& g, int column, int row)

@@ -17,11 +17,7 @@
 namespace {
 
 constexpr int kBlock = 256;
-inline unsigned grid_for(int64_t n, int block = kBlock)
-{
-    int64_t g = (n + block - 1) / block;
-    return (unsigned)(g < 1 ? 1 : g);
-}
+inline unsigned grid_for(int64_t n, int block = kBlock) { return pct_grid_for(n, block); }
 
 // find_MLP_endpoints_GPU, DS.cu:7581-7773; the walk is bounded by 4*(C+R+S)+16 steps (the
 // reference loops forever when the walk never leaves the image, e.g. z clamped at slice 0).

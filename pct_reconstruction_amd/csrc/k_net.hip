@@ -494,7 +494,7 @@ __global__ void k_adam(int64_t n, float lr, float b1, float b2, float eps, float
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-inline unsigned blocks_for(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+inline unsigned blocks_for(int64_t n, int b = kBlock) { return pct_grid_for(n, b); }
 
 }  // namespace
 

@@ -17,13 +17,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
-inline unsigned grid_for(int64_t n, int block = kBlock)
-{
-    int64_t g = (n + block - 1) / block;
-    if (g < 1) g = 1;
-    if (g > 0x7fffffff) g = 0x7fffffff;
-    return (unsigned)g;
-}
+inline unsigned grid_for(int64_t n, int block = kBlock) { return pct_grid_for(n, block); }
 
 // convert_mm_2_cm (DS.cu:4070-4089) and apply_tuv_shifts (:4090-4110): float *= 0.1; float += shift.
 __global__ void k_units(int64_t n, float* t0, float* t1, float* t2, float* t3, float* v0, float* v1, float* v2,

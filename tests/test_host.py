@@ -108,6 +108,56 @@ def test_png_writer_decodes_to_the_quantized_image(tmp_path):
     assert (tmp_path / "x.png").stat().st_size > 65535
 
 
+def test_png_writer_splits_large_images_into_idat_chunks(tmp_path):
+    """A PNG chunk holds at most 2^31 - 1 bytes; the writer splits the zlib stream into IDAT chunks of
+    at most 1 MiB, which decoders concatenate."""
+    import struct
+    from PIL import Image
+    rng = np.random.default_rng(5)
+    img = (rng.random((4, 200, 700)) * 2.0).astype(np.float32)   # 800 scanlines of 1401 bytes: 1.12 MB
+    io.write_image_png(tmp_path / "big.png", img)
+    b = (tmp_path / "big.png").read_bytes()
+    pos, sizes = 8, []
+    while pos < len(b):
+        n, kind = struct.unpack(">I4s", b[pos:pos + 8])
+        if kind == b"IDAT":
+            sizes.append(n)
+        pos += 12 + n
+    assert len(sizes) == 2 and sizes[0] == 1 << 20
+    with Image.open(tmp_path / "big.png") as im:
+        got = np.asarray(im).astype(np.int64)
+    v = img.reshape(800, 700) / 2
+    expect = np.floor(np.clip(v, 0.0, 1.0).astype(np.float32) * np.float32(65535) + np.float32(0.5)).astype(np.int64)
+    assert np.array_equal(got, expect)
+
+
+def test_launch_sizing_beyond_one_dispatch(tmp_path):
+    """Kernels with one work-item per element refuse (zero grid) more work-items than one dispatch holds;
+    grid-stride kernels are capped.  The residual kernels take 16 lanes per history, so above 2^28
+    histories they need the grid-stride form."""
+    import ctypes
+    import subprocess
+    src = tmp_path / "sizing.cpp"
+    src.write_text('#include "launch.h"\n'
+                   'extern "C" unsigned grid_for(long long n, int b) { return pct_grid_for(n, b); }\n'
+                   'extern "C" unsigned grid_capped(long long n, int b, unsigned c) { return pct_grid_capped(n, b, c); }\n')
+    lib = tmp_path / "libsizing.so"
+    csrc = Path(__file__).resolve().parent.parent / "pct_reconstruction_amd" / "csrc"
+    subprocess.run(["g++", "-std=c++17", "-shared", "-fPIC", "-I", str(csrc), "-o", str(lib), str(src)], check=True)
+    L = ctypes.CDLL(str(lib))
+    L.grid_for.argtypes = [ctypes.c_longlong, ctypes.c_int]
+    L.grid_for.restype = ctypes.c_uint
+    L.grid_capped.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_uint]
+    L.grid_capped.restype = ctypes.c_uint
+    assert L.grid_for(0, 256) == 1 and L.grid_for(257, 256) == 2
+    assert L.grid_for((1 << 28) * 16 - 256, 256) == (1 << 24) - 1     # just fits
+    for n in (1 << 28, 300_000_000, 1 << 31):                        # histories x 16 lanes: too many
+        assert L.grid_for(n * 16, 256) == 0
+        g = L.grid_capped(n * 16, 256, 1 << 20)
+        assert g == 1 << 20 and g * 256 < 1 << 32
+    assert L.grid_capped(1000, 256, 1 << 20) == 4
+
+
 def test_generator_is_deterministic_and_physical():
     ph = synth.phantom_cylinder()
     a = synth.generate(ph, synth.beam(), 9, 5000)
