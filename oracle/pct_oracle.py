@@ -400,8 +400,10 @@ def convert_units(P, t, v, u, wepl):
     return t, v, u, mm(np.asarray(wepl, np.float32))
 
 
-def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=False, tvs_sum=TVS_SUM_TILED_FP64):
-    """The reference pipeline on the CPU.  Returns a dict with every intermediate result."""
+def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=False, tvs_sum=TVS_SUM_TILED_FP64,
+                 hull_import=None, x0_import=None):
+    """The reference pipeline on the CPU.  Returns a dict with every intermediate result.  hull_import and
+    x0_import are the images HULL_TYPE = IMPORT and X_0_TYPE = IMPORT read (DS.cu:7358, :8614)."""
     P = derive(P)
     out = {}
     t, v, u, w = convert_units(P, data["t"], data["v"], data["u"], data["wepl"])
@@ -444,7 +446,8 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
             fbp_x0 = out["fbp_median"] = median_filter_2d(P, out["fbp"], P.fbp_med_filter_radius)
         if P.hull_type == 4:
             out["fbp_hull"] = fbp_hull(P, out["fbp"])
-    hull = {1: lambda: out["sc_hull"], 2: lambda: (out["msc_hull"] == 1).astype(np.uint8),
+    hull = {0: lambda: np.asarray(hull_import).reshape(-1) != 0,
+            1: lambda: out["sc_hull"], 2: lambda: (out["msc_hull"] == 1).astype(np.uint8),
             3: lambda: (out["sm_hull"] == 1).astype(np.uint8), 4: lambda: out["fbp_hull"]}[P.hull_type]()
     hull = np.ascontiguousarray(hull, np.uint8)
     out["hull"] = hull
@@ -452,7 +455,9 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     if P.avg_filter_hull and P.hull_avg_filter_radius > 0:
         hull_ep = out["hull_filtered"] = avg_filter_hull(P, hull, P.hull_avg_filter_radius,
                                                          P.hull_avg_filter_threshold)
-    if P.x_0_type == 1:
+    if P.x_0_type == 0:
+        x0 = np.ascontiguousarray(x0_import, np.float32).reshape(-1).copy()
+    elif P.x_0_type == 1:
         x0 = hull.astype(np.float32)
     elif P.x_0_type == 2:
         x0 = fbp_x0.copy()

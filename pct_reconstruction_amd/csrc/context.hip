@@ -635,7 +635,10 @@ extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_vox
 extern "C" int pct_set_hull(pct_ctx* ctx, const uint8_t* hull)
 {
     if (!ctx || !hull) return PCT_E_ARG;
+    // the reference's hull is bool (import_hull, V.cu:6659): any non-zero byte is inside
+    std::vector<uint8_t> b((size_t)ctx->P.num_voxels);
+    for (size_t i = 0; i < b.size(); i++) b[i] = hull[i] != 0;
     PCT_TRY(pct_alloc(ctx, &ctx->hull_import, ctx->P.num_voxels));
-    PCT_CHECK_HIP(ctx, hipMemcpy(ctx->hull_import, hull, ctx->P.num_voxels, hipMemcpyHostToDevice));
+    PCT_CHECK_HIP(ctx, hipMemcpy(ctx->hull_import, b.data(), b.size(), hipMemcpyHostToDevice));
     return PCT_OK;
 }

@@ -1,0 +1,59 @@
+"""Parity of the selectable stage options with the oracle, stage by stage (same bar as test_gpu_parity):
+
+- HULL_TYPE: IMPORT (a caller's hull, import_hull V.cu:6659), SM (hull_selection DS.cu:7351-7393 on the
+  SM counts) and FBP (FBP_image_2_hull_GPU DS.cu:6592-6603);
+- X_0_TYPE: IMPORT, HULL and ZEROS (define_initial_iterate DS.cu:8612-8646);
+- DATA_IN_MM with non-zero t/u/v shifts, the histories entering through projection files
+  (read_data_chunk_v0 DS.cu:4361-4467, convert_mm_2_cm :4070, apply_tuv_shifts :4090).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cases import small_cylinder, small_head
+from test_gpu_parity import _run_and_compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _cylinder_mask(P, radius_cm):
+    C, R, S = P.columns, P.rows, P.slices
+    x = -P.recon_cyl_radius + (np.arange(C) + 0.5) * P.voxel_width
+    y = P.recon_cyl_radius - (np.arange(R) + 0.5) * P.voxel_height
+    r2 = x[None, :] ** 2 + y[:, None] ** 2
+    return np.broadcast_to(r2 <= radius_cm ** 2, (S, R, C)).astype(np.uint8).reshape(-1)
+
+
+def test_parity_import_hull_and_import_x0(oracle):
+    P, data = small_cylinder(n=40000, seed=31, hull_type=0, x_0_type=0)
+    hull = _cylinder_mask(P, 8.6)
+    x0 = np.random.default_rng(7).uniform(0.0, 1.2, P.num_voxels).astype(np.float32)
+    _run_and_compare(P, data, oracle, "import-hull-import-x0", hull_import=hull, x0_import=x0)
+
+
+def test_parity_sm_hull_with_hull_x0(oracle):
+    P, data = small_head(n=50000, seed=13, hull_type=3, x_0_type=1)
+    _run_and_compare(P, data, oracle, "head-SM-hull-x0")
+
+
+def test_parity_fbp_hull_with_zero_x0(oracle):
+    P, data = small_cylinder(n=50000, seed=17, hull_type=4, x_0_type=4)
+    _run_and_compare(P, data, oracle, "cylinder-FBP-hull-zeros")
+
+
+def test_parity_mm_input_with_shifts_through_projection_files(oracle, tmp_path):
+    from pct_reconstruction_amd import io
+    P, data = small_cylinder(n=36000, seed=23, data_in_mm=1, t_shift=0.05, u_shift=-0.03, v_shift=0.02)
+    order = np.argsort(data["angle_deg"], kind="stable")
+    mm = {k: (np.asarray(data[k], np.float32)[..., order] * np.float32(10)).astype(np.float32)
+          for k in ("t", "v", "u", "wepl")}
+    mm["angle_deg"] = np.asarray(data["angle_deg"], np.float32)[order]
+    files = []
+    for a in np.unique(mm["angle_deg"]):
+        sel = mm["angle_deg"] == a
+        f = tmp_path / f"projection_{int(a):03d}.bin"
+        io.write_projection(f, mm["t"][:, sel], mm["v"][:, sel], mm["u"][:, sel], mm["wepl"][sel], angle_deg=float(a))
+        files.append(f)
+    assert np.array_equal(mm["angle_deg"], np.trunc(mm["angle_deg"]))  # integer angles: PCTD v0 files
+    _run_and_compare(P, mm, oracle, "mm-shifts-files", files=files)

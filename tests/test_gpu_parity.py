@@ -26,13 +26,22 @@ def _mismatch(a, b):
     return int(np.sum(a != b))
 
 
-def _run_and_compare(P, data, oracle, case):
+def _run_and_compare(P, data, oracle, case, hull_import=None, x0_import=None, files=None):
+    """Every stage against the oracle.  hull_import / x0_import feed HULL_TYPE / X_0_TYPE = IMPORT; with
+    `files` (projection files holding `data`) the histories enter through the file reader."""
     from pct_reconstruction_amd import Reconstruction
-    ref = oracle.run_pipeline(P, data, iterations=P.iterations, accum=P.drop_accum)
-    ref32 = oracle.run_pipeline(P, data, iterations=P.iterations, accum=0) if P.drop_accum != 0 else ref
+    kw = dict(hull_import=hull_import, x0_import=x0_import)
+    ref = oracle.run_pipeline(P, data, iterations=P.iterations, accum=P.drop_accum, **kw)
+    ref32 = oracle.run_pipeline(P, data, iterations=P.iterations, accum=0, **kw) if P.drop_accum != 0 else ref
     print(f"[{case}] n={data['wepl'].shape[0]} oracle done", flush=True)
     with Reconstruction(P) as r:
-        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        if files is None:
+            r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        else:
+            r.load_projection_files(files)
+        assert r.count("loaded") == data["wepl"].shape[0]
+        if hull_import is not None:
+            r.set_hull(hull_import)
         r.import_and_process_data()
         assert r.count("recon_volume") == ref["h_volume"].shape[1]
         for k, name in enumerate(H_NAMES):
@@ -59,7 +68,7 @@ def _run_and_compare(P, data, oracle, case):
         assert r.count("post_cuts") == ref["h_cut"].shape[1]
         for k, name in enumerate(H_NAMES):
             assert _mismatch(r.histories(name), ref["h_cut"][k]) == 0, name
-        r.generate_preprocessing_images()
+        r.generate_preprocessing_images(x0_import)
         assert _mismatch(r.bins("sinogram"), ref["sinogram"]) == 0
         assert _mismatch(r.bins("sinogram_filtered"), ref["sinogram_filtered"]) == 0
         fbp = r.image("fbp").reshape(-1)
@@ -67,6 +76,8 @@ def _run_and_compare(P, data, oracle, case):
         print(f"[{case}] fbp mismatches {n_fbp} max abs {np.max(np.abs(fbp - ref['fbp']))}", flush=True)
         assert n_fbp == 0
         assert _mismatch(r.image("hull").reshape(-1), ref["hull"].astype(np.float32)) == 0
+        if "fbp_hull" in ref:
+            assert _mismatch(r.image("fbp_hull").reshape(-1), ref["fbp_hull"].astype(np.float32)) == 0
         if "fbp_median" in ref:
             assert _mismatch(r.image("fbp_median").reshape(-1), ref["fbp_median"]) == 0
         if "hull_filtered" in ref:

@@ -188,3 +188,45 @@ def test_rasterize_cylinder_phantom():
     vals = np.unique(img)
     assert all(np.isclose(v, [0.0, 0.3, 1.0, 1.1, 1.7], atol=1e-6).any() for v in vals)
     assert img[4, 50, 50] == pytest.approx(1.0)
+
+
+def _pct_header(magic, version, n, angle, energy, strings):
+    """A projection-file header laid out as count_histories_v0 reads it (DS.cu:3590-3643): magic, version,
+    count, angle, energy, two dates, then three length-prefixed strings."""
+    import struct
+    b = magic + struct.pack("<iiffii", version, n, angle, energy, 1234567, 7654321)
+    for s in strings:
+        b += struct.pack("<i", len(s)) + s
+    return b
+
+
+def test_projection_reader_follows_the_reference_header(tmp_path):
+    """Files built byte by byte from the reference's header description, not by the library's writer: odd
+    string lengths (the data starts at 4 + 7*4 + 2*4 + the three lengths, DS.cu:3643), the 13 arrays in
+    t0..t3, v0..v3, u0..u3, WEPL order (DS.cu:4404-4416), the per-history angles of PCTN / version 1
+    (:4446) and int(header angle) for PCTD version 0 (:4432)."""
+    rng = np.random.default_rng(5)
+    n = 37
+    arrays = rng.normal(size=(13, n)).astype(np.float32)
+    angles = rng.uniform(0, 360, n).astype(np.float32)
+    strings = [b"phantom X", b"src", b"prepared by someone"]
+    f0 = tmp_path / "v0.bin"
+    f0.write_bytes(_pct_header(b"PCTD", 0, n, 36.7, 200.0, strings) + arrays.tobytes())
+    r = io.read_projection(f0)
+    assert r["header_angle_deg"] == np.float32(36.7)
+    for k, name in enumerate(["t", "v", "u"]):
+        assert np.array_equal(r[name], arrays[4 * k:4 * k + 4])
+    assert np.array_equal(r["wepl"], arrays[12])
+    assert (r["angle_deg"] == 36.0).all()
+    f1 = tmp_path / "v1.bin"
+    f1.write_bytes(_pct_header(b"PCTN", 1, n, 0.0, 230.0, strings) + arrays.tobytes() + angles.tobytes())
+    r1 = io.read_projection(f1)
+    assert np.array_equal(r1["t"], arrays[0:4]) and np.array_equal(r1["angle_deg"], angles)
+    f2 = tmp_path / "short.bin"
+    f2.write_bytes(_pct_header(b"PCTD", 0, n, 4.0, 200.0, strings) + arrays.tobytes()[:-4])
+    with pytest.raises(RuntimeError):
+        io.read_projection(f2)
+    f3 = tmp_path / "v2.bin"
+    f3.write_bytes(_pct_header(b"PCTD", 2, n, 4.0, 200.0, strings) + arrays.tobytes())
+    with pytest.raises(RuntimeError):
+        io.read_projection(f3)
