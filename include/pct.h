@@ -272,6 +272,18 @@ int pct_drop_iteration_device(int64_t n, const int64_t* offsets, const uint32_t*
                               const float* wepl, float* x, double* accum, uint32_t* counts, int64_t num_voxels,
                               int block_size, int histories_per_block, int quirk_a3, int min_mlp_length,
                               double lambda, void* hip_stream);
+/*
+ * No counterpart in the reference.  pct_drop_iteration_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same iteration and result, with the per-voxel sums and counts in a library-owned device array of
+ * 2^23 voxels (96 MiB, held by each device the library runs on until the process ends) instead of
+ * caller workspace.  Calls no allocating routine.  Returns PCT_E_LIMIT when num_voxels > 2^23.  The
+ * array is cleared at the start of each call; calls that overlap on one device share it and race.
+ */
+int pct_drop_iteration_device_noalloc(int64_t n, const int64_t* offsets, const uint32_t* voxels,
+                                      const double* chord, const float* wepl, float* x, int64_t num_voxels,
+                                      int block_size, int histories_per_block, int quirk_a3, int min_mlp_length,
+                                      double lambda, void* hip_stream);
 
 /* ---------------------------------------------------------------- I/O ---- */
 /*

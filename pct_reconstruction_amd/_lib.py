@@ -146,6 +146,8 @@ def lib() -> C.CDLL:
         "pct_metrics_json": ([vp], C.c_char_p),
         "pct_drop_iteration_device": ([i64, vp, vp, vp, vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_int, C.c_int,
                                        dbl, vp], C.c_int),
+        "pct_drop_iteration_device_noalloc": ([i64, vp, vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               dbl, vp], C.c_int),
         "pct_read_projection": ([C.c_char_p, P(i64), P(C.c_float), fptr4, fptr4, fptr4, vp, vp], C.c_int),
         "pct_write_projection": ([C.c_char_p, i64, C.c_float, C.c_float, fptr4, fptr4, fptr4, vp, vp], C.c_int),
         "pct_write_image_text": ([C.c_char_p, vp, C.c_int, C.c_int, C.c_int], C.c_int),

@@ -12,6 +12,8 @@
 //   k_drop_apply    x[v] += sum[v]/S[v] for the touched voxels only, and clears sum/S
 //                   (image_update_GPU :9398 does this densely over every voxel).
 // The touched list makes the apply cost proportional to the block's footprint instead of NUM_VOXELS.
+#include <mutex>
+
 #include "internal.h"
 
 namespace {
@@ -281,3 +283,73 @@ extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, d
     return PCT_OK;
 }
 
+
+// ---------------------------------------------------------------- allocation-free DROP iteration ----
+// pct_drop_iteration_device without the workspace arguments: the per-voxel FP64 sums and u32 counts
+// live in `scratch`, a file-scope device array of kNoallocMaxVoxels * 12 bytes (96 MiB) that every
+// device running this library holds until the process ends.  Sums occupy the first 8 * kNoallocMaxVoxels
+// bytes, counts the next 4 * kNoallocMaxVoxels.  Its device address is looked up once per device
+// (hipGetSymbolAddress) and only that address is passed to kernels.  Calls that overlap on one device
+// share the array and race; calls on one stream, or synchronised between streams, do not.
+namespace {
+
+constexpr int64_t kNoallocMaxVoxels = int64_t(1) << 23;
+constexpr int kNoallocMaxDevices = 64;
+
+static __device__ __align__(256) uint8_t scratch[kNoallocMaxVoxels * 12];
+
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kNoallocMaxDevices];
+
+// Device address of the current device's scratch array, looked up on the first call on that device.
+int noalloc_scratch(uint8_t** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return PCT_E_HIP;
+    if (dev < 0 || dev >= kNoallocMaxDevices) return PCT_E_LIMIT;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PCT_E_HIP;
+        g_scratch_ptr[dev] = (uint8_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PCT_OK;
+}
+
+// Zero the sums and counts of the first nv voxels (a fill kernel rather than a memset node, so that
+// the call can be captured in a graph).
+__global__ void k_noalloc_clear(int64_t nv, double* __restrict__ acc, uint32_t* __restrict__ cnt)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
+        acc[v] = 0.0;
+        cnt[v] = 0u;
+    }
+}
+
+}  // namespace
+
+extern "C" int pct_drop_iteration_device_noalloc(int64_t n, const int64_t* offsets, const uint32_t* voxels,
+                                                 const double* chord, const float* wepl, float* x,
+                                                 int64_t num_voxels, int block_size, int histories_per_block,
+                                                 int quirk_a3, int min_mlp_length, double lambda, void* hip_stream)
+{
+    if (n < 0 || !offsets || !voxels || !chord || !wepl || !x || num_voxels <= 0 || block_size <= 0 ||
+        histories_per_block <= 0)
+        return PCT_E_ARG;
+    if (num_voxels > kNoallocMaxVoxels) return PCT_E_LIMIT;
+    uint8_t* base = nullptr;
+    const int rc = noalloc_scratch(&base);
+    if (rc != PCT_OK) return rc;
+    double* acc = (double*)base;
+    uint32_t* cnt = (uint32_t*)(base + 8 * kNoallocMaxVoxels);
+    hipStream_t s = (hipStream_t)hip_stream;
+    hipLaunchKernelGGL(k_noalloc_clear, dim3(pct_grid_capped(num_voxels, kBlock, 4096)), dim3(kBlock), 0, s,
+                       num_voxels, acc, cnt);
+    if (hipGetLastError() != hipSuccess) return PCT_E_HIP;
+    std::string err;
+    return pct_drop_run(s, n, offsets, nullptr, voxels, chord, wepl, x, acc, nullptr, cnt, nullptr, nullptr,
+                        num_voxels, block_size, histories_per_block, quirk_a3, min_mlp_length, lambda,
+                        PCT_ACCUM_FP64, &err);
+}
