@@ -107,21 +107,82 @@ def time_to_residual(r, P, weights, ref_iterations: int) -> dict:
             "speedup": t_fbp / t_net, "weights": weights.name}
 
 
+def fp64_oracle_error(r, P, iterations: int) -> dict:
+    """The image after `iterations` DROP iterations from x0 against the oracle's FP64 DROP (x, sums and scalars in
+    double) over the library's lists, which the parity tests hold bit-identical to the oracle's own."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pct_oracle as O
+    r.reset_iterate()
+    r.DROP(iterations)
+    x = r.image("x").reshape(-1).astype(np.float64)
+    off, vox = r.lists()
+    chord = r.histories("chord")
+    w = r.histories("wepl")
+    x64 = r.image("x0").reshape(-1).astype(np.float64)
+    t0 = time.perf_counter()
+    for _ in range(iterations):
+        x64 = O.drop_iteration_fp64(P, off, vox, chord, w, x64)
+    t = time.perf_counter() - t0
+    del off, vox
+    return {"iterations": iterations, "rel_l2": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+            "max_abs": float(np.abs(x - x64).max()), "oracle_s": t,
+            "sample": f"all {chord.size} reconstruction histories, DROP from x0 in FP64"}
+
+
+def fp64_oracle_error_sampled(P, ph, bm, seed: int, n: int, iterations: int, blocks_per_segment: int) -> dict:
+    """The large-200M path on a sample of its workload: `n` histories with the same geometry, beam and DROP blocks,
+    lists built per segment of `blocks_per_segment` blocks, `iterations` DROP iterations from x0, against the
+    oracle's FP64 DROP over the oracle's own lists."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pct_oracle as O
+    from pct_reconstruction_amd import Reconstruction, synth
+    Q = P.copy()
+    Q.list_segment_histories = blocks_per_segment * Q.drop_block_size
+    data = synth.generate(ph, bm, seed + 1, n)
+    with Reconstruction(Q, device=0) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.preprocessing()
+        r.reconstruction_cuts()
+        r.build_mlp_lists()
+        r.DROP(iterations)
+        x = r.image("x").reshape(-1).astype(np.float64)
+        x0 = r.image("x0").reshape(-1).astype(np.float64)
+        segments = r.metrics()["counts"].get("segments", {}).get("count", 0)
+        names = ["x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "xy_entry_angle", "xz_entry_angle",
+                 "xy_exit_angle", "xz_exit_angle"]
+        h = np.ascontiguousarray(np.stack([r.histories(k) for k in names]))
+        first = np.ascontiguousarray(r.histories("first_voxel"))
+        w = np.ascontiguousarray(r.histories("wepl"))
+    off, vox, chord = O.mlp_lists(Q, O.MlpTables(Q), h, first)
+    x64 = x0
+    for _ in range(iterations):
+        x64 = O.drop_iteration_fp64(Q, off, vox, chord, w, x64)
+    return {"iterations": iterations, "rel_l2": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+            "max_abs": float(np.abs(x - x64).max()),
+            "sample": f"{n} histories of the same geometry and beam ({chord.size} reconstructed, {segments} list "
+                      f"segments of {blocks_per_segment} blocks), DROP from x0, oracle FP64 DROP on its own lists"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="head-50M", choices=["head-50M", "large-200M-segment"],
-                    help="head-50M (the benchmark) or one 25M-history segment of the large-200M run "
-                         "(512x512x128 @ 0.5x0.5x1 mm, 230 MeV, SM hull, block 120000, 12 iterations)")
-    ap.add_argument("--histories", type=int, default=None, help="default: 50M (head-50M), 25M (segment)")
+    ap.add_argument("--workload", default="head-50M", choices=["head-50M", "large-200M", "large-200M-segment"],
+                    help="head-50M (the benchmark); large-200M: 200M histories on 512x512x128 @ 0.5x0.5x1 mm, "
+                         "230 MeV, SM hull, block 120000, 12 iterations, MLP lists and DROP plans built per "
+                         "segment of 115 blocks inside every iteration, learned start; large-200M-segment: one "
+                         "25M-history share of it with resident lists")
+    ap.add_argument("--histories", type=int, default=None, help="default: 50M (head-50M), 200M, 25M (segment)")
+    ap.add_argument("--segment-histories", type=int, default=13_800_000,
+                    help="large-200M: reconstruction histories per list segment (whole DROP blocks)")
     ap.add_argument("--iterations", type=int, default=None, help="DROP iterations per step (default: config)")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="reconstruction histories timed on the CPU (0: all of them)")
     ap.add_argument("--cpu-iterations", type=int, default=2, help="DROP iterations timed on the CPU")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the time-to-residual measurement")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle comparison")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -134,6 +195,12 @@ def main() -> int:
     if args.workload == "head-50M":
         P, ph, bm, n = configs.head_50m(args.histories or 50_000_000)
         workload = "head-50M: 256x256x96 @ ~1 mm, MSC+SM hulls, 3-sigma cuts, FBP x0"
+    elif args.workload == "large-200M":
+        P, ph, bm, n = configs.large_200m(args.histories or 200_000_000)
+        P.list_segment_histories = args.segment_histories
+        workload = (f"large-200M: 512x512x128 @ 0.5x0.5x1 mm, 230 MeV, SM hull, 3-sigma cuts, learned x0 (net on the "
+                    f"iterate after one DROP iteration, kept if the residual drops), MLP lists and DROP plans built "
+                    f"per segment of {args.segment_histories} histories in every iteration")
     else:
         P, ph, bm, n = configs.large_200m(args.histories or 25_000_000)
         workload = ("large-200M segment: 25M of the 200M histories, 512x512x128 @ 0.5x0.5x1 mm, 230 MeV, SM hull, "
@@ -157,10 +224,23 @@ def main() -> int:
     log(f"preprocessing {t_pre:.2f} s; counts {m['counts']}")
     n_recon = r.count("reconstruction")
 
+    learned = None
+    if args.workload == "large-200M":
+        from pct_reconstruction_amd.learned import WarmStartNet
+        learned = WarmStartNet(P.columns, P.rows, P.slices, seed=0)
+        learned.load(str(ROOT / "pct_reconstruction_amd" / "weights" / "warmstart_head50m_x1.bin"))
+    judges = []
+
     def step():
         r.reset_iterate()
         r.build_mlp_lists()
-        r.DROP(iters)
+        if learned is None:
+            r.DROP(iters)
+            return
+        # learned start: one DROP iteration, the net on that iterate (kept only if ||Ax - b|| drops), the rest
+        r.DROP(1)
+        judges.append(learned.apply_iterate(r))
+        r.DROP(iters - 1)
 
     for k in range(args.warmup):
         step()
@@ -170,6 +250,9 @@ def main() -> int:
     t0 = time.perf_counter()
     for k in range(args.steps):
         step()
+        if learned is not None:
+            r.synchronize()
+            log(f"timed step {k + 1}/{args.steps} done")
     r.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"{args.steps} timed steps in {elapsed:.3f} s")
@@ -177,10 +260,14 @@ def main() -> int:
     value = n_recon * iters * args.steps / elapsed
     m = r.metrics()
     entries = r.count("list_entries")
+    segs = m["counts"].get("segments")
 
     # per-kernel timing: one profiled DROP iteration (every launch bracketed by HIP events)
     r.reset_iterate()
     prof = r.profile_drop(1)
+    if segs:
+        log(f"profiled iteration: {segs['count']} segments, list builds {segs['lists_ms']:.0f} ms, "
+            f"DROP blocks {segs['drop_ms']:.0f} ms")
     mlp_ms = m["kernel_ms"]["mlp_lists"]
     nb = max(prof["blocks"], 1)
     # algorithmic bytes per launch of this implementation: rows = list index + x gather per entry, 32 B of
@@ -197,9 +284,15 @@ def main() -> int:
     kernels = {
         "drop_rows": kern(prof["rows_ms"], rows_bytes, iters * nb),
         "drop_cols": kern(prof["cols_ms"], cols_bytes, iters * nb),
-        "mlp_lists": kern(mlp_ms, mlp_bytes, 1),
     }
-    kernels["list_transpose_ms_per_step"] = max(m["stage_ms"]["mlp_build_lists"] - mlp_ms, 0.0)
+    if segs:  # one MLP launch per segment and iteration; the last segment's launch time
+        ns = max(segs["count"], 1)
+        kernels["mlp_lists"] = kern(mlp_ms, mlp_bytes / ns, iters * ns)
+        kernels["list_build_ms_per_iteration"] = segs["lists_ms"]
+        kernels["drop_blocks_ms_per_iteration"] = segs["drop_ms"]
+    else:
+        kernels["mlp_lists"] = kern(mlp_ms, mlp_bytes, 1)
+        kernels["list_transpose_ms_per_step"] = max(m["stage_ms"]["mlp_build_lists"] - mlp_ms, 0.0)
     # The dominant work is the DROP block update, which the reference performs as
     # calculate_x_update_GPU + image_update_GPU and this build as the pair k_drop_rows + k_drop_cols.
     # Its algorithmic bytes per proton-iteration are B_alg = 18 n + 12 (list entry 6 B incl. chord,
@@ -208,19 +301,25 @@ def main() -> int:
     block_ms = prof["rows_ms"] + prof["cols_ms"]
     block_bytes = (18 * prof["gathered"] + 12 * prof["histories"]) / nb
     kernels["drop_block"] = kern(block_ms, block_bytes, iters * nb)
-    traffic = None
+    # HBM traffic per block is not counted in this run: it comes from the committed rocprofv3 --pmc passes
+    # (FETCH_SIZE, WRITE_SIZE) of the same workload and kernels, and is labelled as stored.
+    traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / "drop_block_traffic.json"
-    if tfile.exists():
+    if tfile.exists() and args.workload == "head-50M":
         try:
             t = json.loads(tfile.read_text())
             if t.get("workload_histories") == n:
                 traffic = t.get("hbm_bytes_per_block")
+                traffic_src = f"stored: {tfile.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
         except (OSError, ValueError):
             traffic = None
     roofline = {"bound": "hbm", "kernel": "drop_block (k_drop_rows + k_drop_cols per DROP block)",
                 "achieved": block_bytes / (block_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": block_bytes / (block_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-                "bytes_per_launch": block_bytes, "avg_launch_ms": block_ms}
+                "traffic_source": traffic_src, "bytes_per_launch": block_bytes, "avg_launch_ms": block_ms}
+    # whole-step algorithmic HBM rate: list writes (4 B per entry + 64 B per history) and the DROP blocks
+    step_bytes = iters * (block_bytes * nb) + (iters if segs else 1) * mlp_bytes
+    hbm_step = {"GBs": step_bytes / (ms_per_step * 1e-3) / 1e9, "bytes_per_step": step_bytes}
 
     # time to a fixed residual, learned x0 vs FBP x0 (head-50M only: the shipped net is trained for that grid)
     warm = None
@@ -230,14 +329,29 @@ def main() -> int:
         log(f"time to residual: FBP start {warm['fbp_ms']:.1f} ms, learned start {warm['learned_ms']:.1f} ms "
             f"({warm['iterations_learned']} iterations), speedup {warm['speedup']:.2f}")
 
+    fp64 = None
+    if args.workload == "head-50M" and not args.no_fp64:
+        log("FP64 oracle DROP over the library's lists")
+        fp64 = fp64_oracle_error(r, r.params, 2)
+        log(f"FP64 oracle: rel L2 {fp64['rel_l2']:.3e} after {fp64['iterations']} iterations")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        sample = n_recon if args.cpu_sample <= 0 else min(args.cpu_sample, n_recon)
+        cpu_default = 2_000_000 if args.workload == "large-200M" else 0
+        want = args.cpu_sample if args.cpu_sample > 0 else cpu_default
+        sample = n_recon if want <= 0 else min(want, n_recon)
         log(f"cpu baseline on {sample} histories, {threads} threads")
         cpu = cpu_baseline(r.params, r, sample, threads, iters, max(args.cpu_iterations, 1))
 
     stages = {k: round(v, 3) for k, v in m["stage_ms"].items()}
+    if learned is not None:
+        learned.close()
+    r.close()
+    if args.workload == "large-200M" and not args.no_fp64:
+        log("FP64 oracle on a 1M-history sample of the workload")
+        fp64 = fp64_oracle_error_sampled(P, ph, bm, args.seed, 1_000_000, iters, 2)
+        log(f"FP64 oracle (sample): rel L2 {fp64['rel_l2']:.3e}")
     out = {
         "metric": "MLP+DROP proton histories/s; time to fixed residual: learned x0 vs FBP x0",
         "value": value,
@@ -256,14 +370,16 @@ def main() -> int:
                    "drop_iterations": iters, "drop_block_size": P.drop_block_size},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp64_oracle": fp64,
+        "hbm_step": hbm_step,
         "time_to_residual": warm,
+        "learned_start": judges[-1] if judges else None,
         "kernels": kernels,
         "stages_ms": stages,
         "preprocessing_s": t_pre,
         "counts": m["counts"],
     }
     print(json.dumps(out))
-    r.close()
     return 0
 
 

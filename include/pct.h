@@ -36,6 +36,7 @@ extern "C" {
 enum pct_hull_type { PCT_HULL_IMPORT = 0, PCT_HULL_SC = 1, PCT_HULL_MSC = 2, PCT_HULL_SM = 3, PCT_HULL_FBP = 4 };
 enum pct_filter_type { PCT_FILTER_RAM_LAK = 0, PCT_FILTER_SHEPP_LOGAN = 1, PCT_FILTER_NONE = 2 };
 enum pct_x0_type { PCT_X0_IMPORT = 0, PCT_X0_HULL = 1, PCT_X0_FBP = 2, PCT_X0_HYBRID = 3, PCT_X0_ZEROS = 4 };
+enum pct_mlp_algorithm { PCT_MLP_STANDARD = 0, PCT_MLP_TABULATED = 1 };
 /* DROP per-voxel accumulation precision (the reference uses float atomicAdd, DS.cu:9533):
  * FP32 sums in history order (reference precision), FP64 sums, or FP32 with Kahan compensation. */
 enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1, PCT_ACCUM_KAHAN32 = 2 };
@@ -111,7 +112,7 @@ typedef struct pct_params {
     int32_t avg_filter_fbp;          /* AVG_FILTER_FBP: takes precedence over the median filter (else-if) */
     int32_t avg_filter_hull;         /* AVG_FILTER_HULL: thresholded window average of the hull -> endpoints */
     int32_t hull_avg_filter_radius;  /* HULL_AVG_FILTER_RADIUS */
-    int32_t reserved0;
+    int32_t mlp_algorithm;           /* MLP_ALGORITHM (DS.h:174): 0 = STANDARD (find_MLP_path_GPU :8895), 1 = TABULATED (:9104) */
     double hull_avg_filter_threshold; /* HULL_AVG_FILTER_THRESHOLD (DS.h:483, 0.1) */
     /* total variation superiorization between DROP iterations (DROP_GPU DS.cu:9889-9915, NTVS_iteration
      * :10583, constants DS.h:427-444).  Compiled on in the reference; off here by default (the plain method). */
@@ -122,7 +123,11 @@ typedef struct pct_params {
     int32_t tvs_repetitions;         /* TVS_REPETITIONS (DS.h:443, 5) */
     int32_t tvs_l0;                  /* L_0 (DS.h:437, 0) */
     int32_t tvs_seed;                /* seed of the draw of L (the reference seeds from the clock, :11480) */
-    int32_t reserved1;
+    int32_t list_segment_histories;  /* LIST_SEGMENT_HISTORIES (no reference key; the role of MLP_IN_LOOP and the
+                                      * MAX_*_HISTORIES data segments, DS.cu:9590-9742): 0 = every MLP list and the
+                                      * DROP plan stay on the device; > 0 = they are built per segment of that many
+                                      * reconstruction histories (rounded down to whole DROP blocks) inside each
+                                      * DROP iteration and each residual, so device memory holds one segment */
     double tvs_a;                    /* A (DS.h:436, 0.5): BETA = A^L */
     double tvs_min_eta;              /* TVS_MIN_ETA (DS.h:434, 1e-40): length of the A^L table */
     double tv_threshold;             /* TV_THRESHOLD: the reference's (1/1000) is integer division, 0 */
@@ -337,6 +342,19 @@ int pct_write_image_dat(const char* path, const float* image, int columns, int r
 /* The PNG write_PNG makes from that file with ImageMagick (DS.cu:9969-9971): 16-bit grayscale,
  * COLUMNS x ROWS*SLICES, sample = round(clamp(value/2, 0, 1) * 65535). */
 int pct_write_image_png(const char* path, const float* image, int columns, int rows, int slices);
+/*
+ * The context's images under the reference's output names in directory `dir` (which must exist), as its
+ * stages write them when their WRITE_* switches are on (DS.h:315-322):
+ *   PCT_WRITE_FBP   FBP.txt                       (FBP, DS.cu:6112-6116)
+ *   PCT_WRITE_HULL  hull.txt, hull.dat, hull.png  (hull_selection, DS.cu:7365-7370)
+ *   PCT_WRITE_X0    x_0.txt, x_0.dat, x_0.png     (define_initial_iterate, DS.cu:8637-8641)
+ *   PCT_WRITE_X     x_<k>.txt, x_<k>.dat, x_<k>.png for the current iterate after k DROP iterations since the
+ *                   initial iterate (DROP_GPU, DS.cu:9915-9922)
+ * Text in the array_2_disk layout, .dat and .png as write_PNG (pct_write_image_text/_dat/_png).  Images
+ * whose stage has not run are skipped.
+ */
+enum pct_write_flags { PCT_WRITE_FBP = 1, PCT_WRITE_HULL = 2, PCT_WRITE_X0 = 4, PCT_WRITE_X = 8 };
+int pct_write_reference_layout(pct_ctx* ctx, const char* dir, int flags);
 
 /* ---------------------------------------------------------------- synthetic data ---- */
 /* Phantom made of up to 16 objects; each is a cylinder (kind 0, axis z) or an ellipsoid (kind 1)

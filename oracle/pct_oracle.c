@@ -21,16 +21,35 @@
 #include <omp.h>
 #endif
 
+#ifdef O_FP64
+/* FP64 twin (pct_oracle_fp64.c): every value the reference holds or computes in float is double, the
+ * CUDA float functions are their double versions, and the entry points carry the suffix _f64. */
+typedef double real;
+#define O(name) name##_f64
+#define r_fabs fabs
+#define r_sqrt sqrt
+static inline real f_sq(double x) { return x * x; }
+static inline real f_atan2(real y, real x) { return atan2(y, x); }
+static inline real f_sin(double x) { return sin(x); }
+static inline real f_cos(double x) { return cos(x); }
+static inline real f_tan(double x) { return tan(x); }
+static inline real f_recip(double x) { return 1.0 / x; }
+#else
+typedef float real;
+#define O(name) name
+#define r_fabs fabsf
+#define r_sqrt sqrtf
 static inline float f_sq(double x) { float f = (float)x; return f * f; }
 static inline float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 static inline float f_sin(double x) { return (float)sin((double)(float)x); }
 static inline float f_cos(double x) { return (float)cos((double)(float)x); }
 static inline float f_tan(double x) { return (float)tan((double)(float)x); }
 static inline float f_recip(double x) { float f = (float)x; return 1.0f / f; }
+#endif
 static inline int imax(int a, int b) { return a > b ? a : b; }
 
 /* ------------------------------------------------------------------ parameters (Cfg.h:1745-1789) */
-void o_derive(pct_params* p)
+void O(o_derive)(pct_params* p)
 {
     p->pi = 4 * atan(1.0);                                                   /* DS.h:808 PI */
     p->t_bins = (int32_t)(p->ssd_t_size / p->t_bin_size + 0.5);              /* Cfg.h:1749 */
@@ -99,53 +118,53 @@ static int o_calculate_intercepts(const pct_params* p, double u, double t, doubl
 }
 
 /* recon_volume_intersections_GPU, DS.cu:4890-5021 (identical to the _exact_angles variant :5022) */
-void o_intersections(const pct_params* p, int64_t n, const float* const t[4], const float* const v[4],
-                     const float* const u[4], const float* angle_deg, float* const h[10], uint8_t* missed)
+void O(o_intersections)(const pct_params* p, int64_t n, const real* const t[4], const real* const v[4],
+                     const real* const u[4], const float* angle_deg, real* const h[10], uint8_t* missed)
 {
     const double A2R = p->pi / 180.0, H = p->recon_cyl_height;
-    float *xe = h[0], *ye = h[1], *ze = h[2], *xx = h[3], *yx = h[4], *zx = h[5];
-    float *xya_e = h[6], *xza_e = h[7], *xya_x = h[8], *xza_x = h[9];
+    real *xe = h[0], *ye = h[1], *ze = h[2], *xx = h[3], *yx = h[4], *zx = h[5];
+    real *xya_e = h[6], *xza_e = h[7], *xya_x = h[8], *xza_x = h[9];
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         double rot = (double)angle_deg[i] * A2R;                                         /* :4920 */
         double ut_entry_angle = (double)f_atan2(t[1][i] - t[0][i], u[1][i] - u[0][i]);  /* :4935 */
         double u_entry = 0.0, t_entry = 0.0;
         int entered = o_calculate_intercepts(p, u[1][i], t[1][i], ut_entry_angle, &u_entry, &t_entry);
-        xya_e[i] = (float)(ut_entry_angle + rot);
-        xe[i] = (float)((cos(rot) * u_entry) - (sin(rot) * t_entry));                   /* :4946 */
-        ye[i] = (float)((sin(rot) * u_entry) + (cos(rot) * t_entry));
+        xya_e[i] = (real)(ut_entry_angle + rot);
+        xe[i] = (real)((cos(rot) * u_entry) - (sin(rot) * t_entry));                   /* :4946 */
+        ye[i] = (real)((sin(rot) * u_entry) + (cos(rot) * t_entry));
         double ut_exit_angle = (double)f_atan2(t[3][i] - t[2][i], u[3][i] - u[2][i]);   /* :4951 */
         double u_exit = 0.0, t_exit = 0.0;
         int exited = o_calculate_intercepts(p, u[2][i], t[2][i], ut_exit_angle, &u_exit, &t_exit);
-        xya_x[i] = (float)(ut_exit_angle + rot);
-        xx[i] = (float)((cos(rot) * u_exit) - (sin(rot) * t_exit));
-        yx[i] = (float)((sin(rot) * u_exit) + (cos(rot) * t_exit));
+        xya_x[i] = (real)(ut_exit_angle + rot);
+        xx[i] = (real)((cos(rot) * u_exit) - (sin(rot) * t_exit));
+        yx[i] = (real)((sin(rot) * u_exit) + (cos(rot) * t_exit));
         double uv_entry_slope = (double)((v[1][i] - v[0][i]) / (u[1][i] - u[0][i]));     /* :4967: float / float */
         double uv_exit_slope = (double)((v[3][i] - v[2][i]) / (u[3][i] - u[2][i]));
         xza_e[i] = f_atan2(v[1][i] - v[0][i], u[1][i] - u[0][i]);                       /* :4970 */
         xza_x[i] = f_atan2(v[3][i] - v[2][i], u[3][i] - u[2][i]);
         u_entry = (cos(rot) * (double)xe[i]) + (sin(rot) * (double)ye[i]);              /* :4982 */
         u_exit = (cos(rot) * (double)xx[i]) + (sin(rot) * (double)yx[i]);
-        ze[i] = (float)((double)v[1][i] + uv_entry_slope * (u_entry - (double)u[1][i])); /* :4984 */
-        zx[i] = (float)((double)v[2][i] - uv_exit_slope * ((double)u[2][i] - u_exit));
+        ze[i] = (real)((double)v[1][i] + uv_entry_slope * (u_entry - (double)u[1][i])); /* :4984 */
+        zx[i] = (real)((double)v[2][i] - uv_exit_slope * ((double)u[2][i] - u_exit));
         if (entered && exited) {                                                         /* :4993 */
-            if ((fabsf(ze[i]) < H * 0.5) && (fabsf(zx[i]) > H * 0.5)) {
+            if ((r_fabs(ze[i]) < H * 0.5) && (r_fabs(zx[i]) > H * 0.5)) {
                 double sgn = (double)((zx[i] >= 0) - (zx[i] < 0));
                 double frac = fabs((sgn * H * 0.5 - ze[i]) / (double)(zx[i] - ze[i]));
-                xx[i] = (float)((double)xe[i] + frac * (double)(xx[i] - xe[i]));
-                yx[i] = (float)((double)ye[i] + frac * (double)(yx[i] - ye[i]));
-                zx[i] = (float)(sgn * H * 0.5);
-            } else if (fabsf(ze[i]) > H * 0.5) {
+                xx[i] = (real)((double)xe[i] + frac * (double)(xx[i] - xe[i]));
+                yx[i] = (real)((double)ye[i] + frac * (double)(yx[i] - ye[i]));
+                zx[i] = (real)(sgn * H * 0.5);
+            } else if (r_fabs(ze[i]) > H * 0.5) {
                 entered = 0;
                 exited = 0;
             }
-            if ((fabsf(ze[i]) > H * 0.5) && (fabsf(zx[i]) < H * 0.5)) {                 /* :5003 */
+            if ((r_fabs(ze[i]) > H * 0.5) && (r_fabs(zx[i]) < H * 0.5)) {                 /* :5003 */
                 double sgn_x = (double)((zx[i] >= 0) - (zx[i] < 0));
                 double sgn_e = (double)((ze[i] >= 0) - (ze[i] < 0));
                 double frac = fabs((sgn_x * H * 0.5 - zx[i]) / (double)(zx[i] - ze[i]));
-                xe[i] = (float)((double)xx[i] + frac * (double)(xx[i] - xe[i]));
-                ye[i] = (float)((double)yx[i] + frac * (double)(yx[i] - ye[i]));
-                ze[i] = (float)(sgn_e * H * 0.5);
+                xe[i] = (real)((double)xx[i] + frac * (double)(xx[i] - xe[i]));
+                ye[i] = (real)((double)yx[i] + frac * (double)(yx[i] - ye[i]));
+                ze[i] = (real)(sgn_e * H * 0.5);
             }
             if ((fabsf(t[2][i] - t[1][i]) > 5) || (fabsf(v[2][i] - v[1][i]) > 5)) {     /* :5011 */
                 entered = 0;
@@ -158,7 +177,7 @@ void o_intersections(const pct_params* p, int64_t n, const float* const t[4], co
 
 /* ------------------------------------------------------------------ (a) binning */
 /* binning_GPU, DS.cu:5482-5553 */
-void o_binning(const pct_params* p, int64_t n, const float* const h[10], int32_t* bin_num, uint8_t* in_range)
+void O(o_binning)(const pct_params* p, int64_t n, const real* const h[10], int32_t* bin_num, uint8_t* in_range)
 {
     const double PI = p->pi, R2A = 180.0 / PI, A2R = PI / 180.0;
     const int T = p->t_bins, V = p->v_bins, A = p->angular_bins;
@@ -191,16 +210,16 @@ static inline double o_wrap(double a, double PI)
     return a;
 }
 
-void o_stats_and_cuts(const pct_params* p, int64_t n, const int32_t* bin_num, const uint8_t* in_range,
-                      const float* wepl, const float* const h[10], int32_t* counts_pre, float* const mean[3],
-                      float* const stddev[3], int32_t* counts_post, float* sinogram, uint8_t* failed)
+void O(o_stats_and_cuts)(const pct_params* p, int64_t n, const int32_t* bin_num, const uint8_t* in_range,
+                      const real* wepl, const real* const h[10], int32_t* counts_pre, real* const mean[3],
+                      real* const stddev[3], int32_t* counts_post, real* sinogram, uint8_t* failed)
 {
     const double PI = p->pi;
     const int64_t NB = p->num_bins;
     int32_t* cnt = (int32_t*)calloc((size_t)NB, sizeof(int32_t));
     for (int k = 0; k < 3; k++) {
-        memset(mean[k], 0, (size_t)NB * sizeof(float));
-        memset(stddev[k], 0, (size_t)NB * sizeof(float));
+        memset(mean[k], 0, (size_t)NB * sizeof(real));
+        memset(stddev[k], 0, (size_t)NB * sizeof(real));
     }
     /* binning sums, DS.cu:5518-5546 (float atomics in history order) */
     for (int64_t i = 0; i < n; i++) {
@@ -210,8 +229,8 @@ void o_stats_and_cuts(const pct_params* p, int64_t n, const int32_t* bin_num, co
         double rel_uv = o_wrap((double)(h[9][i] - h[7][i]), PI);
         cnt[b] += 1;
         mean[0][b] += wepl[i];
-        mean[1][b] += (float)rel_ut;
-        mean[2][b] += (float)rel_uv;
+        mean[1][b] += (real)rel_ut;
+        mean[2][b] += (real)rel_uv;
     }
     /* calculate_means_GPU, DS.cu:5644-5654 */
     for (int64_t b = 0; b < NB; b++) {
@@ -238,13 +257,13 @@ void o_stats_and_cuts(const pct_params* p, int64_t n, const int32_t* bin_num, co
     /* calculate_standard_deviations_GPU, DS.cu:5737-5750 (SAMPLE_STD_DEV = true, DS.h:224) */
     for (int64_t b = 0; b < NB; b++) {
         if (cnt[b] > 1) {
-            for (int k = 0; k < 3; k++) stddev[k][b] = sqrtf(stddev[k][b] / (cnt[b] - 1));
+            for (int k = 0; k < 3; k++) stddev[k][b] = r_sqrt(stddev[k][b] / (cnt[b] - 1));
         }
         cnt[b] = 0;
     }
     /* statistical_cuts_GPU, DS.cu:5937-5971; initialize_sinogram :6026 */
-    memset(sinogram, 0, (size_t)NB * sizeof(float));
-    const float k = (float)p->sigmas_2_keep;
+    memset(sinogram, 0, (size_t)NB * sizeof(real));
+    const real k = (real)p->sigmas_2_keep;
     for (int64_t i = 0; i < n; i++) {
         if (!p->quirk_a1_bin0 && !in_range[i]) {
             failed[i] = 1;
@@ -255,7 +274,7 @@ void o_stats_and_cuts(const pct_params* p, int64_t n, const int32_t* bin_num, co
         double rel_uv = o_wrap((double)(h[9][i] - h[7][i]), PI);
         int pass_ut = fabs(rel_ut - mean[1][b]) < (double)(k * stddev[1][b]);
         int pass_uv = fabs(rel_uv - mean[2][b]) < (double)(k * stddev[2][b]);
-        int pass_w = fabsf(mean[0][b] - wepl[i]) <= k * stddev[0][b];
+        int pass_w = r_fabs(mean[0][b] - wepl[i]) <= k * stddev[0][b];
         failed[i] = (uint8_t)(!pass_ut || !pass_uv || !pass_w);
         if (!failed[i]) {
             cnt[b] += 1;
@@ -267,7 +286,7 @@ void o_stats_and_cuts(const pct_params* p, int64_t n, const int32_t* bin_num, co
 }
 
 /* construct_sinogram_GPU, DS.cu:6073-6079 */
-void o_construct_sinogram(const pct_params* p, const int32_t* counts_post, float* sinogram)
+void O(o_construct_sinogram)(const pct_params* p, const int32_t* counts_post, real* sinogram)
 {
     for (int64_t b = 0; b < p->num_bins; b++)
         if (counts_post[b] > 0) sinogram[b] /= counts_post[b];
@@ -281,14 +300,14 @@ static int o_inside_cylinder(const pct_params* p, int column, int row)
     double y = (p->rows / 2 - row - 0.5) * p->voxel_height;
     return (f_sq(x) + f_sq(y)) < f_sq(p->recon_cyl_radius);
 }
-void o_init_hull_u8(const pct_params* p, uint8_t* hull)
+void O(o_init_hull_u8)(const pct_params* p, uint8_t* hull)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     for (int s = 0; s < S; s++)
         for (int r = 0; r < R; r++)
             for (int c = 0; c < C; c++) hull[c + r * C + (int64_t)s * C * R] = (uint8_t)o_inside_cylinder(p, c, r);
 }
-void o_init_hull_i32(const pct_params* p, int32_t* counts)
+void O(o_init_hull_i32)(const pct_params* p, int32_t* counts)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     for (int s = 0; s < S; s++)
@@ -320,12 +339,12 @@ static inline double o_corr(double m, double x, double x0, double y0) { return m
  * Walks stop at a negative voxel coordinate or after 4*(C+R+S)+16 steps (the reference has
  * no such bound; both cases are out-of-bounds or non-terminating there). */
 typedef void (*o_visit_fn)(void* ctx, int64_t voxel);
-static void o_carve_walk(const pct_params* p, const float* const h[10], int64_t i, o_visit_fn visit, void* vctx)
+static void o_carve_walk(const pct_params* p, const real* const h[10], int64_t i, o_visit_fn visit, void* vctx)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     const double X0 = p->x_zero_coordinate, Y0 = p->y_zero_coordinate, Z0 = p->z_zero_coordinate;
     const double VW = p->voxel_width, VH = p->voxel_height, VT = p->voxel_thickness;
-    const float xe = h[0][i], ye = h[1][i], ze = h[2][i], xx = h[3][i], yx = h[4][i], zx = h[5][i];
+    const real xe = h[0][i], ye = h[1][i], ze = h[2][i], xx = h[3][i], yx = h[4][i], zx = h[5][i];
     double x = xe, y = ye, z = ze;
     int x_move = (xe <= xx) - (xe >= xx);
     int y_move = (ye <= yx) - (ye >= yx);
@@ -408,17 +427,17 @@ static void o_carve_walk(const pct_params* p, const float* const h[10], int64_t 
 static void o_visit_sc(void* c, int64_t v) { ((uint8_t*)c)[v] = 0; }
 static void o_visit_count(void* c, int64_t v) { ((int32_t*)c)[v] += 1; }
 
-void o_sc(const pct_params* p, int64_t n, const float* const h[10], const float* wepl, uint8_t* sc_hull)
+void O(o_sc)(const pct_params* p, int64_t n, const real* const h[10], const real* wepl, uint8_t* sc_hull)
 {
     for (int64_t i = 0; i < n; i++)     /* :6686 (missed histories were compacted away) */
         if ((wepl[i] <= p->sc_upper_threshold) && (wepl[i] >= p->sc_lower_threshold)) o_carve_walk(p, h, i, o_visit_sc, sc_hull);
 }
-void o_msc(const pct_params* p, int64_t n, const float* const h[10], const float* wepl, int32_t* counts)
+void O(o_msc)(const pct_params* p, int64_t n, const real* const h[10], const real* wepl, int32_t* counts)
 {
     for (int64_t i = 0; i < n; i++)     /* :6806 */
         if ((wepl[i] <= p->msc_upper_threshold) && (wepl[i] >= p->msc_lower_threshold)) o_carve_walk(p, h, i, o_visit_count, counts);
 }
-void o_sm(const pct_params* p, int64_t n, const float* const h[10], const float* wepl, int32_t* counts)
+void O(o_sm)(const pct_params* p, int64_t n, const real* const h[10], const real* wepl, int32_t* counts)
 {
     for (int64_t i = 0; i < n; i++)     /* :7004 */
         if (wepl[i] >= p->sm_lower_threshold) o_carve_walk(p, h, i, o_visit_count, counts);
@@ -426,8 +445,8 @@ void o_sm(const pct_params* p, int64_t n, const float* const h[10], const float*
 
 static int o_rim(const pct_params* p, int column, int row)
 {
-    float x = (float)((column - p->columns / 2 + 0.5) * p->voxel_width);
-    float y = (float)((p->rows / 2 - row - 0.5) * p->voxel_height);
+    real x = (real)((column - p->columns / 2 + 0.5) * p->voxel_width);
+    real y = (real)((p->rows / 2 - row - 0.5) * p->voxel_height);
     double m = p->voxel_width > p->voxel_height ? p->voxel_width : p->voxel_height;
     return (x * x + y * y) >= f_sq(p->recon_cyl_radius - m / 2);
 }
@@ -445,7 +464,7 @@ static int o_max_diff(const pct_params* p, const int32_t* counts, int c, int r, 
     }
     return md;
 }
-void o_msc_edge(const pct_params* p, const int32_t* counts, int32_t* hull)
+void O(o_msc_edge)(const pct_params* p, const int32_t* counts, int32_t* hull)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     for (int s = 0; s < S; s++) {
@@ -460,7 +479,7 @@ void o_msc_edge(const pct_params* p, const int32_t* counts, int32_t* hull)
             }
     }
 }
-void o_sm_edge(const pct_params* p, const int32_t* counts, int32_t* hull)
+void O(o_sm_edge)(const pct_params* p, const int32_t* counts, int32_t* hull)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     for (int s = 0; s < S; s++) {
@@ -484,7 +503,7 @@ void o_sm_edge(const pct_params* p, const int32_t* counts, int32_t* hull)
 }
 
 /* ------------------------------------------------------------------ (c) FBP */
-void o_filter(const pct_params* p, const float* sino, float* filtered)
+void O(o_filter)(const pct_params* p, const real* sino, real* filtered)
 {
     const int T = p->t_bins, V = p->v_bins, A = p->angular_bins;
     const double PI = p->pi, SR = p->source_radius, TB = p->t_bin_size;
@@ -494,7 +513,7 @@ void o_filter(const pct_params* p, const float* sino, float* filtered)
             double v = (vb - V / 2 + 0.5) * p->v_bin_size;                               /* :6211 */
             int64_t strip = ((int64_t)vb * A + ab) * T;
             for (int tb = 0; tb < T; tb++) {
-                float acc = 0.0f;
+                real acc = 0.0f;
                 for (int tr = 0; tr < T; tr++) {
                     double t = (tr - T / 2 + 0.5) * TB;
                     int sep = tb - tr;
@@ -510,7 +529,7 @@ void o_filter(const pct_params* p, const float* sino, float* filtered)
                     } else {
                         filt = (sep == 0) ? 1.0 / TB : 0.0;  /* UNFILTERED leaves `filtered` unset; identity here */
                     }
-                    acc += (float)(TB * (double)sino[strip + tr] * filt * scale);      /* :6237 */
+                    acc += (real)(TB * (double)sino[strip + tr] * filt * scale);      /* :6237 */
                 }
                 filtered[strip + tb] = acc;
             }
@@ -539,7 +558,7 @@ static inline double o_voxel_pos(int vi, double size, int num, int prog)
 }
 
 /* backprojection_GPU2, DS.cu:6340-6479 */
-void o_backproject(const pct_params* p, const float* sf, float* image)
+void O(o_backproject)(const pct_params* p, const real* sf, real* image)
 {
     const int T = p->t_bins, V = p->v_bins, A = p->angular_bins;
     const double PI = p->pi, SR = p->source_radius, R = p->recon_cyl_radius;
@@ -552,10 +571,10 @@ void o_backproject(const pct_params* p, const float* sf, float* image)
         double y = o_voxel_pos(vy, p->voxel_height, p->rows, -1);
         double z = o_voxel_pos(vz, p->voxel_thickness, p->slices, -1);
         if ((x * x + y * y) > (R * R)) {
-            image[voxel] = (float)p->rsp_air;
+            image[voxel] = (real)p->rsp_air;
             continue;
         }
-        float acc = 0.0f;
+        real acc = 0.0f;
         for (int ab = 0; ab < A; ab++) {
             double ba = ab * delta;
             double u = x * cos(ba) + y * sin(ba);
@@ -570,31 +589,31 @@ void o_backproject(const pct_params* p, const float* sf, float* image)
             double scale = (double)f_sq(SR / (SR + u));
             int64_t bin11 = ((int64_t)v_bin * A + ab) * T + t_bin;
             int64_t bin12 = bin11 + 1, bin21 = bin11 + (int64_t)A * T, bin22 = bin21 + 1;
-            float xterm = 0.0f;
+            real xterm = 0.0f;
             if (t_bin > -1 && t_bin < T - 1) {
                 if (v_bin > -1 && v_bin < V - 1)
-                    xterm = (float)((double)sf[bin11] * (1 - eta) * (1 - eps) + (double)sf[bin12] * eta * (1 - eps) +
+                    xterm = (real)((double)sf[bin11] * (1 - eta) * (1 - eps) + (double)sf[bin12] * eta * (1 - eps) +
                                     (double)sf[bin21] * (1 - eta) * eps + (double)sf[bin22] * eta * eps);
                 else if (v_bin == -1)
-                    xterm = (float)((double)sf[bin21] * (1 - eta) * eps + (double)sf[bin22] * eta * eps);
+                    xterm = (real)((double)sf[bin21] * (1 - eta) * eps + (double)sf[bin22] * eta * eps);
                 else if (v_bin == V - 1)
-                    xterm = (float)((double)sf[bin11] * (1 - eta) * (1 - eps) + (double)sf[bin12] * eta * (1 - eps));
+                    xterm = (real)((double)sf[bin11] * (1 - eta) * (1 - eps) + (double)sf[bin12] * eta * (1 - eps));
             } else if (t_bin == -1) {
                 if (v_bin > -1 && v_bin < V - 1)
-                    xterm = (float)((double)sf[bin12] * eta * (1 - eps) + (double)sf[bin22] * eta * eps);
+                    xterm = (real)((double)sf[bin12] * eta * (1 - eps) + (double)sf[bin22] * eta * eps);
                 else if (v_bin == -1)
-                    xterm = (float)((double)sf[bin22] * eta * eps);
+                    xterm = (real)((double)sf[bin22] * eta * eps);
                 else if (v_bin == V - 1)
-                    xterm = (float)((double)sf[bin12] * eta * (1 - eps));
+                    xterm = (real)((double)sf[bin12] * eta * (1 - eps));
             } else if (t_bin == T - 1) {
                 if (v_bin > -1 && v_bin < V - 1)
-                    xterm = (float)((double)sf[bin11] * (1 - eta) * (1 - eps) + (double)sf[bin21] * (1 - eta) * eps);
+                    xterm = (real)((double)sf[bin11] * (1 - eta) * (1 - eps) + (double)sf[bin21] * (1 - eta) * eps);
                 else if (v_bin == -1)
-                    xterm = (float)((double)sf[bin21] * (1 - eta) * eps);
+                    xterm = (real)((double)sf[bin21] * (1 - eta) * eps);
                 else if (v_bin == V - 1)
-                    xterm = (float)((double)sf[bin11] * (1 - eta) * (1 - eps));
+                    xterm = (real)((double)sf[bin11] * (1 - eta) * (1 - eps));
             }
-            acc += (float)((double)xterm * delta * scale);                               /* :6474 */
+            acc += (real)((double)xterm * delta * scale);                               /* :6474 */
         }
         image[voxel] = acc;
     }
@@ -604,29 +623,29 @@ void o_backproject(const pct_params* p, const float* sf, float* image)
  * element at position (2r+1)^2/2 of their sorted in-slice window, the rest 0. */
 static int o_cmp_float(const void* a, const void* b)
 {
-    float x = *(const float*)a, y = *(const float*)b;
+    real x = *(const real*)a, y = *(const real*)b;
     return (x > y) - (x < y);
 }
-void o_median_filter_2d(const pct_params* p, const float* in, int radius, float* out)
+void O(o_median_filter_2d)(const pct_params* p, const real* in, int radius, real* out)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     const int w = 2 * radius + 1, n = w * w;
-    float* nb = (float*)malloc(sizeof(float) * n);
-    memset(out, 0, sizeof(float) * (size_t)C * R * S);
+    real* nb = (real*)malloc(sizeof(real) * n);
+    memset(out, 0, sizeof(real) * (size_t)C * R * S);
     for (int s = 0; s < S; s++)
         for (int c = radius; c < C - radius; c++)
             for (int r = radius; r < R - radius; r++) {
                 int k = 0;
                 for (int cc = c - radius; cc <= c + radius; cc++)
                     for (int rr = r - radius; rr <= r + radius; rr++) nb[k++] = in[cc + rr * C + (int64_t)s * C * R];
-                qsort(nb, n, sizeof(float), o_cmp_float);
+                qsort(nb, n, sizeof(real), o_cmp_float);
                 out[c + r * C + (int64_t)s * C * R] = nb[n / 2];
             }
     free(nb);
 }
 
 /* averaging_filter_GPU with perform_threshold (DS.cu:7420-7443), on a 0/1 hull */
-void o_avg_filter_hull(const pct_params* p, const uint8_t* in, int radius, double threshold, uint8_t* out)
+void O(o_avg_filter_hull)(const pct_params* p, const uint8_t* in, int radius, double threshold, uint8_t* out)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     for (int s = 0; s < S; s++)
@@ -643,7 +662,7 @@ void o_avg_filter_hull(const pct_params* p, const uint8_t* in, int radius, doubl
 }
 
 /* FBP_image_2_hull_GPU, DS.cu:6592-6603 */
-void o_fbp_hull(const pct_params* p, const float* image, uint8_t* hull)
+void O(o_fbp_hull)(const pct_params* p, const real* image, uint8_t* hull)
 {
     const int C = p->columns, R = p->rows, S = p->slices;
     const double RC = p->recon_cyl_radius;
@@ -675,7 +694,7 @@ static double o_poly(const double* c, const float* P, double u, const int* d)
     double in5 = u * c[5] * P[5] / d[5];
     return a0 + u * (a1 + u * (a2 + u * (a3 + u * (a4 + in5))));
 }
-int o_tables_create(const pct_params* p, o_tables* t)
+int O(o_tables_create)(const pct_params* p, o_tables* t)
 {
     memset(t, 0, sizeof(*t));
     t->n_trig = (int64_t)p->trig_table_elements + 1;
@@ -712,7 +731,7 @@ int o_tables_create(const pct_params* p, o_tables* t)
     }
     return 0;
 }
-void o_tables_destroy(o_tables* t)
+void O(o_tables_destroy)(o_tables* t)
 {
     free(t->sin_t); free(t->cos_t); free(t->scat);
     free(t->p12); free(t->p23); free(t->p34); free(t->p26); free(t->p312);
@@ -818,7 +837,7 @@ static int o_find_endpoint(const pct_params* p, const uint8_t* hull, double x_st
 }
 
 /* collect_MLP_endpoints_GPU_nobool, DS.cu:7819-7854 */
-void o_endpoints(const pct_params* p, const uint8_t* hull, int64_t n, float* const h[10], uint32_t* first_voxel)
+void O(o_endpoints)(const pct_params* p, const uint8_t* hull, int64_t n, real* const h[10], uint32_t* first_voxel)
 {
     const int C = p->columns, R = p->rows;
 #pragma omp parallel for schedule(dynamic, 256)
@@ -829,8 +848,8 @@ void o_endpoints(const pct_params* p, const uint8_t* hull, int64_t n, float* con
         int exited = o_find_endpoint(p, hull, h[3][i], h[4][i], h[5][i], h[8][i], h[9][i], &xo, &yo, &zo, &vxi, &vyi, &vzi, 0);
         if (entered && exited) {
             first_voxel[i] = (uint32_t)(vx + C * vy + R * C * vz);
-            h[0][i] = (float)xi; h[1][i] = (float)yi; h[2][i] = (float)zi;
-            h[3][i] = (float)xo; h[4][i] = (float)yo; h[5][i] = (float)zo;
+            h[0][i] = (real)xi; h[1][i] = (real)yi; h[2][i] = (real)zi;
+            h[3][i] = (real)xo; h[4][i] = (real)yo; h[5][i] = (real)zo;
         } else {
             first_voxel[i] = (uint32_t)p->num_voxels;
         }
@@ -839,19 +858,19 @@ void o_endpoints(const pct_params* p, const uint8_t* hull, int64_t n, float* con
 
 /* ------------------------------------------------------------------ (d) MLP */
 /* EffectiveChordLength_GPU, DS.cu:8873-8894 */
-double o_chord_length(const pct_params* p, double abs_angle_t, double abs_angle_v)
+double O(o_chord_length)(const pct_params* p, double abs_angle_t, double abs_angle_v)
 {
     const double PI = p->pi, VW = p->voxel_width;
     double eff_t = abs_angle_t - ((int)(abs_angle_t / (PI / 2))) * (PI / 2);
     double eff_v = fabs(abs_angle_v);
     double sf = p->mlp_u_step / VW;
-    float s2t = f_sin(2 * eff_t), ct = f_cos(eff_t), st = f_sin(eff_t);
-    float cs = ct + st;                                        /* cosf + sinf: float add */
+    real s2t = f_sin(2 * eff_t), ct = f_cos(eff_t), st = f_sin(eff_t);
+    real cs = ct + st;                                        /* cosf + sinf: float add */
     double c2d = (1 / 3.0) * ((sf * sf * s2t - 6) / (sf * s2t - (double)(2 * cs)) + sf * sf * s2t / (double)(2 * cs));
     double mpw = VW / (double)cs;
     double hf = p->voxel_thickness / mpw;
     sf = p->mlp_u_step / mpw;
-    float s2v = f_sin(2 * eff_v), cv = f_cos(eff_v), sv = f_sin(eff_v);
+    real s2v = f_sin(2 * eff_v), cv = f_cos(eff_v), sv = f_sin(eff_v);
     double den = hf * cv + sv;
     double c3d = (1 / 3.0) * ((sf * sf * s2v - 6 * hf) / (sf * s2v - 2 * den) + sf * sf * s2v / (2 * den));
     return VW * c2d * c3d;
@@ -860,7 +879,7 @@ double o_chord_length(const pct_params* p, double abs_angle_t, double abs_angle_
 /* find_MLP_path_GPU_tabulated, DS.cu:9104-9262.  Writes up to cap voxels into path, returns the count
  * (or -1 past cap) and the chord.  Voxels whose linear index is >= NUM_VOXELS are not listed (the
  * reference reads and writes out of bounds there). */
-static int64_t o_mlp_path(const pct_params* p, const o_tables* tb, const float* const h[10], int64_t i, uint32_t first,
+static int64_t o_mlp_path(const pct_params* p, const o_tables* tb, const real* const h[10], int64_t i, uint32_t first,
                           uint32_t* path, int64_t cap, double* chord, double* rec, int64_t rec_cap, int64_t* n_rec)
 {
     const double PI = p->pi, STEP = p->mlp_u_step;
@@ -898,7 +917,7 @@ static int64_t o_mlp_path(const pct_params* p, const o_tables* tb, const float* 
     int64_t n = 0;
     uint32_t last = first;
     path[n++] = first;
-    *chord = o_chord_length(p, (xy_entry + xy_exit) / 2.0, (xz_entry + xz_exit) / 2.0);
+    *chord = O(o_chord_length)(p, (xy_entry + xy_exit) / 2.0, (xz_entry + xz_exit) / 2.0);
     while (depth > STEP) {
         double u1_12 = tb->p12[u1i], u1_23 = tb->p23[u1i];
         double st1 = tb->p312[u1i], st1th1 = tb->p26[u1i], sth1 = u1_12;
@@ -948,7 +967,139 @@ static int64_t o_mlp_path(const pct_params* p, const o_tables* tb, const float* 
     return n;
 }
 
-int64_t o_mlp_lists(const pct_params* p, const o_tables* t, int64_t n, const float* const h[10],
+/* powf(x, 3) of a double argument: x rounded to float, cubed (the float-argument double cube, rounded). */
+#ifdef O_FP64
+static inline real f_cube(double x) { return x * x * x; }
+#else
+static inline float f_cube(double x) { double f = (double)(float)x; return (float)(f * f * f); }
+#endif
+
+/* find_MLP_path_GPU (MLP_ALGORITHM = STANDARD), DS.cu:8895-9103: the same MLP as the tabulated path with the
+ * scattering moments evaluated from the polynomial at every step, the scattering coefficients from log(),
+ * the rotation from cos()/sin() of the entry angle, and the explicit 2x2 matrix products
+ * (R_0 = [1 u_1; 0 1], R_1 = [1 u_2-u_1; 0 1], R_1^T) kept as written.  Differences from the tabulated
+ * path that the reference has: the chord is taken from the angles before the +pi turn (:8921 precedes
+ * :8945), u_1 counts up by MLP_U_STEP and the loop runs while u_1 < u_2 - MLP_U_STEP (:8978), and the
+ * powers of u_1 (powf(u_1, 2), powf(u_1, 3)), the two scattering coefficients and the off-diagonal
+ * squares in the determinants are float (:8985-9005).  Voxels >= NUM_VOXELS are not listed, as in
+ * o_mlp_path. */
+static int64_t o_mlp_path_standard(const pct_params* p, const real* const h[10], int64_t i, uint32_t first,
+                                   uint32_t* path, int64_t cap, double* chord, double* rec, int64_t rec_cap,
+                                   int64_t* n_rec)
+{
+    const double PI = p->pi, STEP = p->mlp_u_step, E0 = p->e_0, X0 = p->x_0_rad_length;
+    const int C = p->columns, R = p->rows;
+    const int64_t NV = p->num_voxels;
+    double c[6];
+    float Pw[6];
+    o_poly_coeffs(c, Pw);
+    static const int d12[6] = {1, 2, 3, 4, 5, 6}, d23[6] = {2, 3, 4, 5, 6, 7}, d34[6] = {3, 4, 5, 6, 7, 8};
+    static const int d26[6] = {2, 6, 12, 20, 30, 42}, d312[6] = {3, 12, 30, 60, 105, 168};
+    double x_in = h[0][i], y_in = h[1][i], z_in = h[2][i], x_out = h[3][i], y_out = h[4][i], z_out = h[5][i];
+    double xy_entry = h[6][i], xz_entry = h[7][i], xy_exit = h[8][i], xz_exit = h[9][i];
+    *chord = O(o_chord_length)(p, (xy_entry + xy_exit) / 2.0, (xz_entry + xz_exit) / 2.0);         /* :8921 */
+    double u_in = cos(xy_entry) * x_in + sin(xy_entry) * y_in;                                 /* :8938 */
+    double u_out = cos(xy_entry) * x_out + sin(xy_entry) * y_out;
+    double t_in = cos(xy_entry) * y_in - sin(xy_entry) * x_in;
+    double t_out = cos(xy_entry) * y_out - sin(xy_entry) * x_out;
+    if (u_in > u_out) {                                                                         /* :8945 */
+        xy_entry += PI;
+        xy_exit += PI;
+        u_in = cos(xy_entry) * x_in + sin(xy_entry) * y_in;
+        u_out = cos(xy_entry) * x_out + sin(xy_entry) * y_out;
+        t_in = cos(xy_entry) * y_in - sin(xy_entry) * x_in;
+        t_out = cos(xy_entry) * y_out - sin(xy_entry) * x_out;
+    }
+    const double T0[2] = {t_in, 0.0}, T2[2] = {t_out, xy_exit - xy_entry};
+    const double V0[2] = {z_in, 0.0}, V2[2] = {z_out, xz_exit - xz_entry};
+    const double u_0 = 0.0;
+    double u_1 = STEP;
+    const double u_2 = fabs(u_out - u_in);
+    const double pre1 = u_2 * u_2 * u_2 * o_poly(c, Pw, u_2, d312);                            /* :8974 */
+    const double pre2 = u_2 * u_2 * o_poly(c, Pw, u_2, d26);
+    const double pre3 = u_2 * o_poly(c, Pw, u_2, d12);
+    double R0[4] = {1.0, 0.0, 0.0, 1.0}, R1[4] = {1.0, 0.0, 0.0, 1.0}, R1T[4] = {1.0, 0.0, 0.0, 1.0};
+    int64_t n = 0;
+    uint32_t last = first;
+    path[n++] = first;
+    while (u_1 < u_2 - STEP) {
+        R0[1] = u_1;
+        R1[1] = u_2 - u_1;
+        R1T[2] = u_2 - u_1;
+        double k1 = (double)f_sq(E0 * (1 + 0.038 * log((u_1 - u_0) / X0))) / X0;              /* :8985 */
+        double st1 = k1 * ((double)f_cube(u_1) * o_poly(c, Pw, u_1, d312));
+        double st1th1 = k1 * ((double)f_sq(u_1) * o_poly(c, Pw, u_1, d26));
+        double sth1 = k1 * (u_1 * o_poly(c, Pw, u_1, d12));
+        double det1 = st1 * sth1 - (double)f_sq(st1th1);
+        double S1[4] = {sth1 / det1, -st1th1 / det1, -st1th1 / det1, st1 / det1};
+        double k2 = (double)f_sq(E0 * (1 + 0.038 * log((u_2 - u_1) / X0))) / X0;              /* :8997 */
+        double cm1 = u_1 * o_poly(c, Pw, u_1, d12);
+        double cm2 = (double)f_sq(u_1) * o_poly(c, Pw, u_1, d23);
+        double cm3 = (double)f_cube(u_1) * o_poly(c, Pw, u_1, d34);
+        double st2 = k2 * (pre1 - (double)f_sq(u_2) * cm1 + 2 * u_2 * cm2 - cm3);
+        double st2th2 = k2 * (pre2 - u_2 * cm1 + cm2);
+        double sth2 = k2 * (pre3 - cm1);
+        double det2 = st2 * sth2 - (double)f_sq(st2th2);
+        double S2[4] = {sth2 / det2, -st2th2 / det2, -st2th2 / det2, st2 / det2};
+        double c13_1 = S2[0] * R1[0] + S2[1] * R1[2];                                           /* :9012 */
+        double c13_2 = S2[2] * R1[0] + S2[3] * R1[2];
+        double c24_1 = S2[0] * R1[1] + S2[1] * R1[3];
+        double c24_2 = S2[2] * R1[1] + S2[3] * R1[3];
+        double ft[4];
+        ft[0] = S1[0] + R1T[0] * c13_1 + R1T[1] * c13_2;
+        ft[1] = S1[1] + R1T[0] * c24_1 + R1T[1] * c24_2;
+        ft[2] = S1[2] + R1T[2] * c13_1 + R1T[3] * c13_2;
+        ft[3] = S1[3] + R1T[2] * c24_1 + R1T[3] * c24_2;
+        double detft = ft[0] * ft[3] - ft[1] * ft[2];
+        double tmp = ft[0];
+        ft[0] = ft[3] / detft;
+        ft[1] = -ft[1] / detft;
+        ft[2] = -ft[2] / detft;
+        ft[3] = tmp / detft;
+        double sc1 = R0[0] * T0[0] + R0[1] * T0[1];                                             /* :9031 */
+        double sc2 = R0[2] * T0[0] + R0[3] * T0[1];
+        double sc3 = S2[0] * T2[0] + S2[1] * T2[1];
+        double sc4 = S2[2] * T2[0] + S2[3] * T2[1];
+        double s0 = S1[0] * sc1 + S1[1] * sc2 + R1T[0] * sc3 + R1T[1] * sc4;
+        double s1 = S1[2] * sc1 + S1[3] * sc2 + R1T[2] * sc3 + R1T[3] * sc4;
+        double t_1 = ft[0] * s0 + ft[1] * s1;
+        sc1 = R0[0] * V0[0] + R0[1] * V0[1];                                                    /* :9050 */
+        sc2 = R0[2] * V0[0] + R0[3] * V0[1];
+        sc3 = S2[0] * V2[0] + S2[1] * V2[1];
+        sc4 = S2[2] * V2[0] + S2[3] * V2[1];
+        s0 = S1[0] * sc1 + S1[1] * sc2 + R1T[0] * sc3 + R1T[1] * sc4;
+        s1 = S1[2] * sc1 + S1[3] * sc2 + R1T[2] * sc3 + R1T[3] * sc4;
+        double v_1 = ft[0] * s0 + ft[1] * s1;
+        double x_1 = (cos(xy_entry) * (u_in + u_1)) - (sin(xy_entry) * t_1);                   /* :9067 */
+        double y_1 = (sin(xy_entry) * (u_in + u_1)) + (cos(xy_entry) * t_1);
+        int vx = o_calc_voxel(p->x_zero_coordinate, x_1, p->voxel_width);
+        int vy = o_calc_voxel(p->y_zero_coordinate, y_1, p->voxel_height);
+        int vz = o_calc_voxel(p->z_zero_coordinate, v_1, p->voxel_thickness);
+        int64_t voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
+        if (rec && *n_rec < rec_cap) {
+            double* r = rec + 6 * *n_rec;
+            r[0] = u_in + u_1; r[1] = t_1; r[2] = v_1; r[3] = x_1; r[4] = y_1; r[5] = (double)voxel;
+        }
+        if (rec) *n_rec += 1;
+        if (voxel != (int64_t)last && voxel < NV) {
+            if (n >= cap) return -1;
+            path[n++] = (uint32_t)voxel;
+            last = (uint32_t)voxel;
+        }
+        u_1 += STEP;
+    }
+    return n;
+}
+
+static int64_t o_mlp_any(const pct_params* p, const o_tables* tb, const real* const h[10], int64_t i, uint32_t first,
+                         uint32_t* path, int64_t cap, double* chord, double* rec, int64_t rec_cap, int64_t* n_rec)
+{
+    if (p->mlp_algorithm == PCT_MLP_STANDARD)
+        return o_mlp_path_standard(p, h, i, first, path, cap, chord, rec, rec_cap, n_rec);
+    return o_mlp_path(p, tb, h, i, first, path, cap, chord, rec, rec_cap, n_rec);
+}
+
+int64_t O(o_mlp_lists)(const pct_params* p, const o_tables* t, int64_t n, const real* const h[10],
                     const uint32_t* first_voxel, int64_t* offsets, uint32_t* voxels, int64_t capacity, double* chord)
 {
     /* Histories are split into contiguous chunks; each chunk's lists go to a private buffer that is
@@ -970,7 +1121,7 @@ int64_t o_mlp_lists(const pct_params* p, const o_tables* t, int64_t n, const flo
         int64_t used = 0;
         for (int64_t i = a; i < b && buf; i++) {
             double L;
-            int64_t k = o_mlp_path(p, t, h, i, first_voxel[i], buf + used, cap - used, &L, NULL, 0, NULL);
+            int64_t k = o_mlp_any(p, t, h, i, first_voxel[i], buf + used, cap - used, &L, NULL, 0, NULL);
             if (k < 0) { failed = 1; break; }
             used += k;
             offsets[i + 1] = k;   /* count for now */
@@ -998,15 +1149,15 @@ int64_t o_mlp_lists(const pct_params* p, const o_tables* t, int64_t n, const flo
     return total;
 }
 
-int64_t o_mlp_steps(const pct_params* p, const o_tables* t, const float* hist10, uint32_t first, double* rec,
+int64_t O(o_mlp_steps)(const pct_params* p, const o_tables* t, const real* hist10, uint32_t first, double* rec,
                     int64_t rec_cap, uint32_t* path, int64_t path_cap, int64_t* path_len, double* chord)
 {
-    float one[10];
-    const float* h[10];
+    real one[10];
+    const real* h[10];
     for (int k = 0; k < 10; k++) { one[k] = hist10[k]; h[k] = &one[k]; }
     int64_t n_rec = 0;
     double L = 0.0;
-    int64_t n = o_mlp_path(p, t, h, 0, first, path, path_cap, &L, rec, rec_cap, &n_rec);
+    int64_t n = o_mlp_any(p, t, h, 0, first, path, path_cap, &L, rec, rec_cap, &n_rec);
     if (path_len) *path_len = n;
     if (chord) *chord = L;
     return n_rec;
@@ -1021,8 +1172,8 @@ static void o_drop_block_range(const pct_params* p, int64_t n, int64_t start, in
 
 static double o_row_chord(const double* chord, int64_t i) { return chord[i]; }
 
-void o_drop_iteration(const pct_params* p, int64_t n, const int64_t* offsets, const uint32_t* voxels,
-                      const double* chord, const float* wepl, float* x, int accum, int nthreads)
+void O(o_drop_iteration)(const pct_params* p, int64_t n, const int64_t* offsets, const uint32_t* voxels,
+                      const double* chord, const real* wepl, real* x, int accum, int nthreads)
 {
     const int64_t NV = p->num_voxels;
     const double lambda = p->lambda;
@@ -1093,7 +1244,7 @@ void o_drop_iteration(const pct_params* p, int64_t n, const int64_t* offsets, co
         }
         for (int64_t v = 0; v < NV; v++) {                                               /* :9398-9419 */
             if (S[v] > 0) {
-                if (accum == PCT_ACCUM_FP64) x[v] += (float)(U64[v] / (double)S[v]);
+                if (accum == PCT_ACCUM_FP64) x[v] += (real)(U64[v] / (double)S[v]);
                 else x[v] += U32[v] / (float)S[v];
                 U32[v] = 0.0f;
                 K32[v] = 0.0f;
@@ -1108,8 +1259,8 @@ void o_drop_iteration(const pct_params* p, int64_t n, const int64_t* offsets, co
     free(S);
 }
 
-void o_drop_iteration_fp64(const pct_params* p, int64_t n, const int64_t* offsets, const uint32_t* voxels,
-                           const double* chord, const float* wepl, double* x)
+void O(o_drop_iteration_fp64)(const pct_params* p, int64_t n, const int64_t* offsets, const uint32_t* voxels,
+                           const double* chord, const real* wepl, double* x)
 {
     const int64_t NV = p->num_voxels;
     double* U = (double*)calloc((size_t)NV, sizeof(double));
@@ -1137,8 +1288,8 @@ void o_drop_iteration_fp64(const pct_params* p, int64_t n, const int64_t* offset
     free(S);
 }
 
-double o_residual(int64_t n, const int64_t* offsets, const uint32_t* voxels, const double* chord, const float* wepl,
-                  const float* x, int min_len)
+double O(o_residual)(int64_t n, const int64_t* offsets, const uint32_t* voxels, const double* chord, const real* wepl,
+                  const real* x, int min_len)
 {
     double acc = 0.0;
     for (int64_t i = 0; i < n; i++) {

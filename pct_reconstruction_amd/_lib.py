@@ -20,6 +20,8 @@ ERRORS = {-1: "invalid argument", -2: "stage called out of order", -3: "HIP runt
 HULL_IMPORT, HULL_SC, HULL_MSC, HULL_SM, HULL_FBP = 0, 1, 2, 3, 4
 FILTER_RAM_LAK, FILTER_SHEPP_LOGAN, FILTER_NONE = 0, 1, 2
 X0_IMPORT, X0_HULL, X0_FBP, X0_HYBRID, X0_ZEROS = 0, 1, 2, 3, 4
+MLP_STANDARD, MLP_TABULATED = 0, 1
+WRITE_FBP, WRITE_HULL, WRITE_X0, WRITE_X = 1, 2, 4, 8
 ACCUM_FP32, ACCUM_FP64, ACCUM_KAHAN32 = 0, 1, 2
 DROP_SPATIAL_ORDER, DROP_XCD_SLABS, DROP_BRICKS, DROP_COLS_LDS, DROP_COLS_SLICED = 1, 2, 4, 8, 16
 DROP_DEFAULT_FLAGS = DROP_SPATIAL_ORDER | DROP_BRICKS | DROP_COLS_SLICED  # the context's default (internal.h)
@@ -54,10 +56,10 @@ class Params(C.Structure):
         ("poly_table_range", _D), ("poly_table_step", _D), ("e_0", _D), ("x_0_rad_length", _D), ("rsp_air", _D),
         ("source_radius", _D), ("data_in_mm", _I), ("quirk_a1_bin0", _I), ("quirk_a3_drop_floor", _I),
         ("quirk_voxel_decode", _I), ("drop_accum", _I), ("median_filter_fbp", _I), ("fbp_med_filter_radius", _I),
-        ("avg_filter_fbp", _I), ("avg_filter_hull", _I), ("hull_avg_filter_radius", _I), ("reserved0", _I),
+        ("avg_filter_fbp", _I), ("avg_filter_hull", _I), ("hull_avg_filter_radius", _I), ("mlp_algorithm", _I),
         ("hull_avg_filter_threshold", _D),
         ("tvs_on", _I), ("tvs_first", _I), ("tvs_conditioned", _I), ("tvs_old", _I), ("tvs_repetitions", _I),
-        ("tvs_l0", _I), ("tvs_seed", _I), ("reserved1", _I), ("tvs_a", _D), ("tvs_min_eta", _D),
+        ("tvs_l0", _I), ("tvs_seed", _I), ("list_segment_histories", _I), ("tvs_a", _D), ("tvs_min_eta", _D),
         ("tv_threshold", _D),
         # derived
         ("t_bins", _I), ("v_bins", _I), ("angular_bins", _I), ("num_bins", _I), ("slices", _I), ("num_voxels", _I),
@@ -146,6 +148,7 @@ def lib() -> C.CDLL:
         "pct_metrics_json": ([vp], C.c_char_p),
         "pct_drop_iteration_device": ([i64, vp, vp, vp, vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_int, C.c_int,
                                        dbl, vp], C.c_int),
+        "pct_write_reference_layout": ([vp, C.c_char_p, C.c_int], C.c_int),
         "pct_drop_iteration_device_noalloc": ([i64, vp, vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_int, C.c_int,
                                                dbl, vp], C.c_int),
         "pct_read_projection": ([C.c_char_p, P(i64), P(C.c_float), fptr4, fptr4, fptr4, vp, vp], C.c_int),

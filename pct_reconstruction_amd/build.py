@@ -71,7 +71,8 @@ def build_oracle() -> Path:
     ORACLE_LIB.parent.mkdir(exist_ok=True)
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     _run(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-Wall", "-Wextra",
-          "-o", str(tmp), str(ORACLE_DIR / "pct_oracle.c"), str(ORACLE_DIR / "pct_net_oracle.c"),
+          "-o", str(tmp), str(ORACLE_DIR / "pct_oracle.c"), str(ORACLE_DIR / "pct_oracle_fp64.c"),
+          str(ORACLE_DIR / "pct_net_oracle.c"),
           str(ORACLE_DIR / "pct_tvs_oracle.c"), "-lm"])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB

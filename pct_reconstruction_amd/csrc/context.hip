@@ -189,6 +189,10 @@ int pct_get_count(pct_ctx* ctx, int kind, int64_t* out)
     case PCT_N_CURRENT: *out = ctx->n_hist; break;
     case PCT_N_LIST_ENTRIES: {
         if (ctx->stage < ST_LISTS || ctx->n_recon == 0) { *out = 0; break; }
+        if (pct_segment_histories(ctx) > 0) {  // lists of the segments of the last DROP iteration
+            *out = ctx->seg.entries;
+            break;
+        }
         // sum of the per-history counts
         std::vector<uint32_t> c(ctx->n_recon);
         PCT_CHECK_HIP(ctx, hipMemcpy(c.data(), ctx->list_cnt, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -468,7 +472,10 @@ int pct_get_histories(pct_ctx* ctx, int kind, void* out, size_t capacity)
     else if (kind == PCT_H_GANTRY_ANGLE) src = ctx->gantry;
     else if (kind == PCT_H_BIN_NUM) src = ctx->bin_num;
     else if (kind == PCT_H_FIRST_VOXEL) src = ctx->first_voxel;
-    else if (kind == PCT_H_PATH_COUNT) src = ctx->stage >= ST_LISTS ? ctx->list_cnt : nullptr;
+    else if ((kind == PCT_H_PATH_COUNT || kind == PCT_H_CHORD) && pct_segment_histories(ctx) > 0) {
+        ctx->err = "path counts and chords are built per segment (LIST_SEGMENT_HISTORIES) and not kept";
+        return PCT_E_STATE;
+    } else if (kind == PCT_H_PATH_COUNT) src = ctx->stage >= ST_LISTS ? ctx->list_cnt : nullptr;
     else if (kind == PCT_H_CHORD) {
         src = ctx->stage >= ST_LISTS ? ctx->chord : nullptr;
         esz = 8;
@@ -486,6 +493,10 @@ int pct_get_lists(pct_ctx* ctx, int64_t* offsets, size_t offsets_cap, uint32_t* 
     if (!ctx || !offsets) return PCT_E_ARG;
     if (ctx->stage < ST_LISTS) {
         ctx->err = "lists not built";
+        return PCT_E_STATE;
+    }
+    if (pct_segment_histories(ctx) > 0) {
+        ctx->err = "lists are built per segment (LIST_SEGMENT_HISTORIES) and not kept";
         return PCT_E_STATE;
     }
     const int64_t n = ctx->n_recon;
@@ -537,9 +548,19 @@ const char* pct_metrics_json(pct_ctx* ctx)
     s += buf;
     size_t mem_free = 0, mem_total = 0;
     (void)hipMemGetInfo(&mem_free, &mem_total);
-    std::snprintf(buf, sizeof(buf), ", \"list_entries\": %lld, \"list_capacity\": %lld, \"device_bytes_used\": %lld",
-                  (long long)ctx->list_entries, (long long)ctx->list_cap, (long long)(mem_total - mem_free));
+    const bool segmented = ctx->stage >= ST_LISTS && pct_segment_histories(ctx) > 0;
+    std::snprintf(buf, sizeof(buf),
+                  ", \"list_entries\": %lld, \"list_slots\": %lld, \"list_capacity\": %lld, \"device_bytes_used\": %lld",
+                  (long long)(segmented ? ctx->seg.entries : ctx->list_entries), (long long)ctx->list_slots,
+                  (long long)ctx->list_cap, (long long)(mem_total - mem_free));
     s += buf;
+    if (segmented) {
+        std::snprintf(buf, sizeof(buf),
+                      ", \"segments\": {\"histories\": %lld, \"count\": %lld, \"lists_ms\": %.3f, \"drop_ms\": %.3f}",
+                      (long long)pct_segment_histories(ctx), (long long)ctx->seg.segments, ctx->seg.lists_ms,
+                      ctx->seg.drop_ms);
+        s += buf;
+    }
     s += "}, \"stage_ms\": {";
     for (int k = 0; k <= TM_DROP; k++) {
         std::snprintf(buf, sizeof(buf), "%s\"%s\": %.4f", k ? ", " : "", kTimerName[k], ctx->t_stage[k]);
@@ -629,6 +650,29 @@ extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_vox
     ctx->drop_row_lanes = g;
     ctx->drop_col_voxels = v;
     ctx->drop_flags = drop_flags;
+    return PCT_OK;
+}
+
+// Output files of the reference's stages (array_2_disk + write_PNG) for the context's images.
+extern "C" int pct_write_reference_layout(pct_ctx* ctx, const char* dir, int flags)
+{
+    if (!ctx || !dir || (flags & ~(PCT_WRITE_FBP | PCT_WRITE_HULL | PCT_WRITE_X0 | PCT_WRITE_X))) return PCT_E_ARG;
+    const pct_params& P = ctx->P;
+    std::vector<float> img((size_t)P.num_voxels);
+    auto write = [&](int kind, const std::string& base, bool with_png) -> int {
+        PCT_TRY(pct_get_image(ctx, kind, img.data()));
+        const std::string stem = std::string(dir) + "/" + base;
+        int rc = pct_write_image_text((stem + ".txt").c_str(), img.data(), P.columns, P.rows, P.slices);
+        if (rc == PCT_OK && with_png) rc = pct_write_image_dat((stem + ".dat").c_str(), img.data(), P.columns, P.rows, P.slices);
+        if (rc == PCT_OK && with_png) rc = pct_write_image_png((stem + ".png").c_str(), img.data(), P.columns, P.rows, P.slices);
+        if (rc != PCT_OK) ctx->err = "writing " + stem + " failed";
+        return rc;
+    };
+    if ((flags & PCT_WRITE_FBP) && ctx->fbp) PCT_TRY(write(PCT_IMG_FBP, "FBP", false));
+    if ((flags & PCT_WRITE_HULL) && ctx->hull && ctx->stage >= ST_HULL_SEL) PCT_TRY(write(PCT_IMG_HULL, "hull", true));
+    if ((flags & PCT_WRITE_X0) && ctx->x0 && ctx->stage >= ST_X0) PCT_TRY(write(PCT_IMG_X0, "x_0", true));
+    if ((flags & PCT_WRITE_X) && ctx->x && ctx->stage >= ST_X0)
+        PCT_TRY(write(PCT_IMG_X, "x_" + std::to_string(ctx->drop_k), true));
     return PCT_OK;
 }
 

@@ -123,6 +123,12 @@ struct TvsRecord {
     double ms;                   // device time of the iteration
 };
 
+// Totals of one segmented DROP iteration, summed over its segments.
+struct SegStats {
+    int64_t segments = 0, entries = 0, histories = 0, gathered = 0, scattered = 0, touched = 0;
+    double lists_ms = 0, drop_ms = 0;  // host wall time in the list builds (incl. plans) and in the DROP blocks
+};
+
 struct pct_ctx {
     pct_params P;
     int device = 0;
@@ -193,6 +199,7 @@ struct pct_ctx {
     double* res_r2 = nullptr;        // residual: per history (b - a.x)^2, and their sum
     int64_t res_cap = 0;
     int64_t list_cap = 0, list_entries = 0, list_hist_cap = 0;
+    int64_t list_slots = 0;  // capacity slots of the last list build (1 + u-steps per history)
     DevTables T;
 
     // DROP plan (transposed lists) and the scratch of the atomic DROP kernels
@@ -221,6 +228,11 @@ struct pct_ctx {
     // timing (ms) of the last run of each stage, and of the last MLP list kernel launch
     double t_stage[16] = {};
     double t_mlp_kernel = 0;
+
+    // list segments (LIST_SEGMENT_HISTORIES): the lists and the DROP plan describe one segment at a time
+    bool seg_inside = false;      // a SegmentView is active: the per-history arrays are a segment's window
+    bool seg_lists_only = false;  // build a segment's lists without its DROP plan (residual)
+    SegStats seg;                 // totals of the last segmented DROP iteration
 };
 
 // allocation helpers
@@ -285,6 +297,11 @@ void pct_drop_plan_free(DropPlan* plan);
 int pct_drop_enter(pct_ctx* ctx, bool enter);  // image into (true) / out of the DROP layout
 int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing);
 int pct_stage_residual(pct_ctx* ctx, double* norm);
+// list segments: the segment length in histories (0 = no segments), and the segmented DROP iteration and
+// residual over every segment (segments.hip)
+int64_t pct_segment_histories(const pct_ctx* ctx);
+int pct_stage_drop_segments(pct_ctx* ctx, int iterations, DropTiming* timing);
+int pct_stage_residual_segments(pct_ctx* ctx, double* norm);
 int pct_tvs_iteration(pct_ctx* ctx, int k);               // NTVS_iteration for DROP iteration k (1-based)
 int pct_stage_total_variation(pct_ctx* ctx, double* tv);  // TV of ctx->x, fixed-order FP64 sum
 

@@ -215,6 +215,7 @@ int pct_drop_run(hipStream_t s, int64_t n, const int64_t* off, const uint32_t* c
 
 int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing)
 {
+    if (!ctx->seg_inside && pct_segment_histories(ctx) > 0) return pct_stage_drop_segments(ctx, iterations, timing);
     if (!ctx->plan.ready || ctx->plan.flags != ctx->drop_flags) PCT_TRY(pct_drop_prepare(ctx));
     if (!ctx->P.tvs_on) {
         PCT_TRY(pct_drop_enter(ctx, true));
@@ -273,10 +274,17 @@ extern "C" int pct_drop_profile(pct_ctx* ctx, int iterations, double* rows_ms, d
     }
     DropTiming t;
     PCT_TRY(pct_stage_drop(ctx, iterations, &t));
-    PCT_TRY(pct_drop_entries(ctx, histories, gathered, scattered));
-    int64_t u = 0;
-    for (const auto& bk : ctx->plan.blocks) u += bk.n_vox;
-    *touched = u;
+    if (pct_segment_histories(ctx) > 0) {  // totals of the last iteration's segments
+        *histories = ctx->seg.histories;
+        *gathered = ctx->seg.gathered;
+        *scattered = ctx->seg.scattered;
+        *touched = ctx->seg.touched;
+    } else {
+        PCT_TRY(pct_drop_entries(ctx, histories, gathered, scattered));
+        int64_t u = 0;
+        for (const auto& bk : ctx->plan.blocks) u += bk.n_vox;
+        *touched = u;
+    }
     *rows_ms = t.scatter_ms / (t.scatter_launches ? t.scatter_launches : 1);
     *cols_ms = t.apply_ms / (t.apply_launches ? t.apply_launches : 1);
     *launches = t.scatter_launches / iterations;

@@ -1068,6 +1068,7 @@ int pct_drop_iterate_plan(pct_ctx* ctx, DropTiming* timing)
 // With the brick layout prepared, the lists' brick indices gather from a brick-ordered copy of x.
 int pct_stage_residual(pct_ctx* ctx, double* norm)
 {
+    if (!ctx->seg_inside && pct_segment_histories(ctx) > 0) return pct_stage_residual_segments(ctx, norm);
     const int64_t n = ctx->n_recon;
     hipStream_t s = ctx->stream;
     DropPlan& D = ctx->plan;

@@ -317,6 +317,221 @@ __global__ void __launch_bounds__(kBlock) k_mlp_lists(int64_t n, MlpArgs a, H10 
     }
 }
 
+// ---------------------------------------------------------------- untabulated MLP ----
+// find_MLP_path_GPU (MLP_ALGORITHM = STANDARD), DS.cu:8895-9103.  The scattering polynomial of every
+// Sigma term is evaluated in the kernel: a0 + u(a1 + u(a2 + u(a3 + u(a4 + ((u c5) P5) / d5)))), where the
+// a_k = (c_k P_k) / d_k are formed on the host (the A_k_OVER_m macros of DS.h:756-781 expand that way).
+struct PolyFam {
+    double a[5];
+    double d5;
+};
+struct StdArgs {
+    DevGeom g;
+    double step, e0, x0, c5, p5;
+    PolyFam f12, f23, f34, f26, f312;
+    int max_steps;
+};
+
+__device__ __forceinline__ double std_poly(const StdArgs& a, const PolyFam& f, double u)
+{
+    double in5 = u * a.c5 * a.p5 / f.d5;
+    return f.a[0] + u * (f.a[1] + u * (f.a[2] + u * (f.a[3] + u * (f.a[4] + in5))));
+}
+
+// powf(x, 3) of a double argument: x rounded to float, cubed in double (exact square, one rounding), rounded.
+__device__ __forceinline__ float std_cubef(double x)
+{
+    double f = (double)(float)x;
+    return (float)(f * f * f);
+}
+
+// Rotation into the entry frame with cos()/sin() of the entry angle (DS.cu:8938-8955).
+struct StdFrame {
+    double c, s, u_in, u_out, t_in, t_out, xy_entry, xy_exit;
+};
+__device__ __forceinline__ StdFrame std_frame(double PI, double x_in, double y_in, double x_out, double y_out,
+                                              double xy_entry, double xy_exit)
+{
+    StdFrame f;
+    f.c = cos(xy_entry);
+    f.s = sin(xy_entry);
+    f.u_in = f.c * x_in + f.s * y_in;
+    f.u_out = f.c * x_out + f.s * y_out;
+    f.t_in = f.c * y_in - f.s * x_in;
+    f.t_out = f.c * y_out - f.s * x_out;
+    if (f.u_in > f.u_out) {
+        xy_entry += PI;
+        xy_exit += PI;
+        f.c = cos(xy_entry);
+        f.s = sin(xy_entry);
+        f.u_in = f.c * x_in + f.s * y_in;
+        f.u_out = f.c * x_out + f.s * y_out;
+        f.t_in = f.c * y_in - f.s * x_in;
+        f.t_out = f.c * y_out - f.s * x_out;
+    }
+    f.xy_entry = xy_entry;
+    f.xy_exit = xy_exit;
+    return f;
+}
+
+// capacity: 1 + the u-steps of `while (u_1 < u_2 - MLP_U_STEP)` with u_1 += MLP_U_STEP (DS.cu:8978)
+__global__ void k_mlp_capacity_std(int64_t n, StdArgs a, H10 hp, int64_t* cap)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    StdFrame f = std_frame(a.g.PI, hp.h[0][i], hp.h[1][i], hp.h[3][i], hp.h[4][i], hp.h[6][i], hp.h[8][i]);
+    const double u_2 = fabs(f.u_out - f.u_in);
+    double u_1 = a.step;
+    int64_t steps = 0;
+    while (u_1 < u_2 - a.step && steps < a.max_steps) {
+        u_1 += a.step;
+        steps++;
+    }
+    cap[i] = 1 + steps;
+}
+
+// One history per lane, grid-stride.  FP64 throughout, with the reference's float powf roundings.
+__global__ void __launch_bounds__(kBlock) k_mlp_lists_std(int64_t n, StdArgs a, H10 hp,
+                                                          const uint32_t* __restrict__ first,
+                                                          const int64_t* __restrict__ off,
+                                                          uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ vox,
+                                                          double* __restrict__ chord, int32_t* overflow)
+{
+    const DevGeom& g = a.g;
+    const double STEP = a.step, E0 = a.e0, X0 = a.x0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double x_in = hp.h[0][i], y_in = hp.h[1][i], z_in = hp.h[2][i];
+        const double x_out = hp.h[3][i], y_out = hp.h[4][i], z_out = hp.h[5][i];
+        const double xy_e0 = hp.h[6][i], xz_entry = hp.h[7][i], xy_x0 = hp.h[8][i], xz_exit = hp.h[9][i];
+        // the chord comes from the angles before the +pi turn (DS.cu:8921 precedes :8945)
+        chord[i] = chord_length(g.PI, g.VW, g.VT, STEP, (xy_e0 + xy_x0) / 2.0, (xz_entry + xz_exit) / 2.0);
+        StdFrame f = std_frame(g.PI, x_in, y_in, x_out, y_out, xy_e0, xy_x0);
+        const double T0_0 = f.t_in, T0_1 = 0.0, T2_0 = f.t_out, T2_1 = f.xy_exit - f.xy_entry;
+        const double V0_0 = z_in, V0_1 = 0.0, V2_0 = z_out, V2_1 = xz_exit - xz_entry;
+        const double u_0 = 0.0, u_2 = fabs(f.u_out - f.u_in);
+        const double pre1 = u_2 * u_2 * u_2 * std_poly(a, a.f312, u_2);
+        const double pre2 = u_2 * u_2 * std_poly(a, a.f26, u_2);
+        const double pre3 = u_2 * std_poly(a, a.f12, u_2);
+        const double u2_sq = (double)pct_sqf(u_2);
+        const double R0_0 = 1.0, R0_2 = 0.0, R0_3 = 1.0, R1_0 = 1.0, R1_2 = 0.0, R1_3 = 1.0;
+        const double R1T_0 = 1.0, R1T_1 = 0.0, R1T_3 = 1.0;
+        const int64_t base = off[i], capn = off[i + 1] - off[i];
+        uint32_t last = first[i];
+        int64_t cnt = 0;
+        vox[base + cnt++] = last;
+        double u_1 = STEP;
+        int k = 0;
+        while (u_1 < u_2 - STEP) {
+            if (k >= a.max_steps) {
+                atomicOr(overflow, 1);
+                break;
+            }
+            const double R0_1 = u_1, R1_1 = u_2 - u_1, R1T_2 = u_2 - u_1;
+            const double k1 = (double)pct_sqf(E0 * (1 + 0.038 * log((u_1 - u_0) / X0))) / X0;
+            const double st1 = k1 * ((double)std_cubef(u_1) * std_poly(a, a.f312, u_1));
+            const double st1th1 = k1 * ((double)pct_sqf(u_1) * std_poly(a, a.f26, u_1));
+            const double sth1 = k1 * (u_1 * std_poly(a, a.f12, u_1));
+            const double det1 = st1 * sth1 - (double)pct_sqf(st1th1);
+            const double S1_0 = sth1 / det1, S1_1 = -st1th1 / det1, S1_2 = -st1th1 / det1, S1_3 = st1 / det1;
+            const double k2 = (double)pct_sqf(E0 * (1 + 0.038 * log((u_2 - u_1) / X0))) / X0;
+            const double cm1 = u_1 * std_poly(a, a.f12, u_1);
+            const double cm2 = (double)pct_sqf(u_1) * std_poly(a, a.f23, u_1);
+            const double cm3 = (double)std_cubef(u_1) * std_poly(a, a.f34, u_1);
+            const double st2 = k2 * (pre1 - u2_sq * cm1 + 2 * u_2 * cm2 - cm3);
+            const double st2th2 = k2 * (pre2 - u_2 * cm1 + cm2);
+            const double sth2 = k2 * (pre3 - cm1);
+            const double det2 = st2 * sth2 - (double)pct_sqf(st2th2);
+            const double S2_0 = sth2 / det2, S2_1 = -st2th2 / det2, S2_2 = -st2th2 / det2, S2_3 = st2 / det2;
+            const double c13_1 = S2_0 * R1_0 + S2_1 * R1_2;
+            const double c13_2 = S2_2 * R1_0 + S2_3 * R1_2;
+            const double c24_1 = S2_0 * R1_1 + S2_1 * R1_3;
+            const double c24_2 = S2_2 * R1_1 + S2_3 * R1_3;
+            double ft0 = S1_0 + R1T_0 * c13_1 + R1T_1 * c13_2;
+            double ft1 = S1_1 + R1T_0 * c24_1 + R1T_1 * c24_2;
+            double ft2 = S1_2 + R1T_2 * c13_1 + R1T_3 * c13_2;
+            double ft3 = S1_3 + R1T_2 * c24_1 + R1T_3 * c24_2;
+            const double detft = ft0 * ft3 - ft1 * ft2;
+            const double tmp = ft0;
+            ft0 = ft3 / detft;
+            ft1 = -ft1 / detft;
+            ft2 = -ft2 / detft;
+            ft3 = tmp / detft;
+            double sc1 = R0_0 * T0_0 + R0_1 * T0_1;
+            double sc2 = R0_2 * T0_0 + R0_3 * T0_1;
+            double sc3 = S2_0 * T2_0 + S2_1 * T2_1;
+            double sc4 = S2_2 * T2_0 + S2_3 * T2_1;
+            double s0 = S1_0 * sc1 + S1_1 * sc2 + R1T_0 * sc3 + R1T_1 * sc4;
+            double s1 = S1_2 * sc1 + S1_3 * sc2 + R1T_2 * sc3 + R1T_3 * sc4;
+            const double t_1 = ft0 * s0 + ft1 * s1;
+            sc1 = R0_0 * V0_0 + R0_1 * V0_1;
+            sc2 = R0_2 * V0_0 + R0_3 * V0_1;
+            sc3 = S2_0 * V2_0 + S2_1 * V2_1;
+            sc4 = S2_2 * V2_0 + S2_3 * V2_1;
+            s0 = S1_0 * sc1 + S1_1 * sc2 + R1T_0 * sc3 + R1T_1 * sc4;
+            s1 = S1_2 * sc1 + S1_3 * sc2 + R1T_2 * sc3 + R1T_3 * sc4;
+            const double v_1 = ft0 * s0 + ft1 * s1;
+            const double x_1 = (f.c * (f.u_in + u_1)) - (f.s * t_1);
+            const double y_1 = (f.s * (f.u_in + u_1)) + (f.c * t_1);
+            const int vx = pct_calc_voxel(g.X0, x_1, g.VW);
+            const int vy = pct_calc_voxel(g.Y0, y_1, g.VH);
+            const int vz = pct_calc_voxel(g.Z0, v_1, g.VT);
+            const int64_t voxel = vx + (int64_t)vy * g.C + (int64_t)vz * g.C * g.R;
+            if (voxel != (int64_t)last && voxel < g.NV) {
+                if (cnt >= capn) {
+                    atomicOr(overflow, 2);
+                    break;
+                }
+                vox[base + cnt++] = (uint32_t)voxel;
+                last = (uint32_t)voxel;
+            }
+            u_1 += STEP;
+            k++;
+        }
+        cnt_out[i] = (uint32_t)cnt;
+    }
+}
+
+// total of the per-history list counts (one 64-bit atomic per wavefront)
+__global__ void k_count_entries(int64_t n, const uint32_t* __restrict__ cnt, unsigned long long* total)
+{
+    unsigned long long part = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) part += cnt[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    if ((threadIdx.x & 63) == 0 && part) atomicAdd(total, part);
+}
+
+StdArgs std_args(const pct_params& P, int max_steps)
+{
+    StdArgs a;
+    a.g = pct_geom(P);
+    a.step = P.mlp_u_step;
+    a.e0 = P.e_0;
+    a.x0 = P.x_0_rad_length;
+    const double c[6] = {7.457, 4.548, -5.777, 1.301, -9.228, 2.687};  // DS.h:749-754
+    const float Pw[6] = {std::pow(10.0f, -6.0f), std::pow(10.0f, -7.0f), std::pow(10.0f, -8.0f),
+                         std::pow(10.0f, -8.0f), std::pow(10.0f, -10.0f), std::pow(10.0f, -11.0f)};
+    auto fam = [&](const int* d) {
+        PolyFam f;
+        for (int k = 0; k < 5; k++) f.a[k] = c[k] * Pw[k] / d[k];
+        f.d5 = d[5];
+        return f;
+    };
+    static const int d12[6] = {1, 2, 3, 4, 5, 6}, d23[6] = {2, 3, 4, 5, 6, 7}, d34[6] = {3, 4, 5, 6, 7, 8};
+    static const int d26[6] = {2, 6, 12, 20, 30, 42}, d312[6] = {3, 12, 30, 60, 105, 168};
+    a.c5 = c[5];
+    a.p5 = Pw[5];
+    a.f12 = fam(d12);
+    a.f23 = fam(d23);
+    a.f34 = fam(d34);
+    a.f26 = fam(d26);
+    a.f312 = fam(d312);
+    a.max_steps = max_steps;
+    return a;
+}
+
 }  // namespace
 
 // reconstruction_cuts (DS.cu:8296): endpoints on the selected hull, keep histories that hit it
@@ -383,6 +598,13 @@ int pct_stage_build_lists(pct_ctx* ctx)
     hipStream_t s = ctx->stream;
     const int64_t n = ctx->n_hist;
     PCT_TRY(upload_tables(ctx));
+    if (!ctx->seg_inside && pct_segment_histories(ctx) > 0) {
+        // lists are built per segment inside each DROP iteration and residual (segments.hip)
+        ctx->list_entries = 0;
+        ctx->plan.ready = false;
+        ctx->stage = ST_LISTS;
+        return PCT_OK;
+    }
     MlpArgs a;
     a.g = pct_geom(P);
     a.trig_min = P.trig_table_min;
@@ -410,7 +632,11 @@ int pct_stage_build_lists(pct_ctx* ctx)
     // capacities -> exclusive scan -> offsets
     int64_t* capv = ctx->cap_scratch;
     PCT_CHECK_HIP(ctx, hipMemsetAsync(capv, 0, (n + 1) * sizeof(int64_t), s));
-    if (n > 0)
+    const bool standard = P.mlp_algorithm == PCT_MLP_STANDARD;
+    const StdArgs sa = std_args(P, 4 * a.n_steps);
+    if (n > 0 && standard)
+        hipLaunchKernelGGL(k_mlp_capacity_std, dim3(grid_for(n)), dim3(kBlock), 0, s, n, sa, hp, capv);
+    else if (n > 0)
         hipLaunchKernelGGL(k_mlp_capacity, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, hp, ctx->T.sin_t,
                            ctx->T.cos_t, capv);
     size_t need = 0;
@@ -443,9 +669,14 @@ int pct_stage_build_lists(pct_ctx* ctx)
         PCT_CHECK_HIP(ctx, hipEventCreate(&e0));
         PCT_CHECK_HIP(ctx, hipEventCreate(&e1));
         PCT_CHECK_HIP(ctx, hipEventRecord(e0, s));
-        hipLaunchKernelGGL(k_mlp_lists, dim3(blocks), dim3(kBlock), lds, s, n, a, hp, ctx->first_voxel, ctx->T.sin_t,
-                           ctx->T.cos_t, ctx->T.scat, ctx->T.p12, ctx->T.p26, ctx->T.p312, ctx->T.step_terms,
-                           ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord, ctx->list_overflow);
+        if (standard)
+            hipLaunchKernelGGL(k_mlp_lists_std, dim3(blocks), dim3(kBlock), 0, s, n, sa, hp, ctx->first_voxel,
+                               ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord, ctx->list_overflow);
+        else
+            hipLaunchKernelGGL(k_mlp_lists, dim3(blocks), dim3(kBlock), lds, s, n, a, hp, ctx->first_voxel,
+                               ctx->T.sin_t, ctx->T.cos_t, ctx->T.scat, ctx->T.p12, ctx->T.p26, ctx->T.p312,
+                               ctx->T.step_terms, ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord,
+                               ctx->list_overflow);
         PCT_CHECK_HIP(ctx, hipGetLastError());
         PCT_CHECK_HIP(ctx, hipEventRecord(e1, s));
         PCT_CHECK_HIP(ctx, hipEventSynchronize(e1));
@@ -462,10 +693,20 @@ int pct_stage_build_lists(pct_ctx* ctx)
         ctx->err = "MLP list capacity exceeded (code " + std::to_string(overflow) + ")";
         return PCT_E_LIMIT;
     }
-    ctx->list_entries = total;
+    // entries actually listed (the capacity `total` counts every u-step)
+    unsigned long long* d_total = (unsigned long long*)ctx->cap_scratch;  // free after the scan
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
+    if (n > 0)
+        hipLaunchKernelGGL(k_count_entries, dim3(pct_grid_capped(n, kBlock, 2048)), dim3(kBlock), 0, s, n,
+                           ctx->list_cnt, d_total);
+    unsigned long long listed = 0;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&listed, d_total, sizeof(listed), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    ctx->list_entries = (int64_t)listed;
+    ctx->list_slots = total;
     ctx->stage = ST_LISTS;
     // the DROP blocks' transposed lists belong to the list build
     ctx->plan.ready = false;
-    PCT_TRY(pct_drop_prepare(ctx));
+    if (!ctx->seg_lists_only) PCT_TRY(pct_drop_prepare(ctx));
     return PCT_OK;
 }

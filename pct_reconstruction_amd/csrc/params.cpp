@@ -74,7 +74,7 @@ extern "C" void pct_params_default(pct_params* p)
     p->avg_filter_fbp = 0;
     p->avg_filter_hull = 0;
     p->hull_avg_filter_radius = 0;
-    p->reserved0 = 0;
+    p->mlp_algorithm = PCT_MLP_TABULATED;  // DS.h:174
     p->hull_avg_filter_threshold = 0.1;
     // total variation superiorization: DS.h:427-444 except TVS_ON (off here, the plain method)
     p->tvs_on = 0;
@@ -84,7 +84,7 @@ extern "C" void pct_params_default(pct_params* p)
     p->tvs_repetitions = 5;
     p->tvs_l0 = 0;
     p->tvs_seed = 0;
-    p->reserved1 = 0;
+    p->list_segment_histories = 0;
     p->tvs_a = 0.5;
     p->tvs_min_eta = 1e-40;
     p->tv_threshold = 0.0;
@@ -124,12 +124,12 @@ const Key kKeys[] = {
     PK_B("QUIRK_VOXEL_DECODE", quirk_voxel_decode), PK_I("DROP_ACCUM", drop_accum),
     PK_B("MEDIAN_FILTER_FBP", median_filter_fbp), PK_I("FBP_MED_FILTER_RADIUS", fbp_med_filter_radius),
     PK_B("AVG_FILTER_FBP", avg_filter_fbp), PK_B("AVG_FILTER_HULL", avg_filter_hull),
-    PK_I("HULL_AVG_FILTER_RADIUS", hull_avg_filter_radius),
+    PK_I("HULL_AVG_FILTER_RADIUS", hull_avg_filter_radius), PK_I("MLP_ALGORITHM", mlp_algorithm),
     PK_D("HULL_AVG_FILTER_THRESHOLD", hull_avg_filter_threshold),
     PK_B("TVS_ON", tvs_on), PK_B("TVS_FIRST", tvs_first), PK_B("TVS_CONDITIONED", tvs_conditioned),
     PK_B("TVS_OLD", tvs_old), PK_I("TVS_REPETITIONS", tvs_repetitions), PK_I("L_0", tvs_l0),
     PK_I("TVS_SEED", tvs_seed), PK_D("A", tvs_a), PK_D("TVS_MIN_ETA", tvs_min_eta),
-    PK_D("TV_THRESHOLD", tv_threshold),
+    PK_D("TV_THRESHOLD", tv_threshold), PK_I("LIST_SEGMENT_HISTORIES", list_segment_histories),
 };
 
 // Keys of settings.cfg that configure parts of the reference outside this path (I/O directories,
@@ -138,7 +138,7 @@ const Key kKeys[] = {
 const char* kIgnored[] = {
     "PROJECTION_DATA_DIR", "PREPROCESSING_DIR", "RECONSTRUCTION_DIR", "PATH_2_PCT_DATA_DIR", "OBJECT", "RUN_DATE",
     "RUN_NUMBER", "PROJECTION_DATA_DATE", "PREPROCESS_DATE", "USER_NAME", "DATA_TYPE", "RECONSTRUCTION_METHOD",
-    "ENDPOINTS_TX_MODE", "DROP_TX_MODE", "ENDPOINTS_ALG", "MLP_ALGORITHM", "NUM_SCANS", "T_BINS", "V_BINS",
+    "ENDPOINTS_TX_MODE", "DROP_TX_MODE", "ENDPOINTS_ALG", "NUM_SCANS", "T_BINS", "V_BINS",
     "RECON_CYL_HEIGHT", "SLICES", "IMAGE_WIDTH", "IMAGE_HEIGHT", "IMAGE_THICKNESS", "VOXEL_WIDTH", "VOXEL_HEIGHT",
     "VOXEL_THICKNESS", "ETA", "PSI_SIGN", "THREADS_PER_BLOCK", "ENDPOINTS_PER_BLOCK", "ENDPOINTS_PER_THREAD",
     "HISTORIES_PER_THREAD", "VOXELS_PER_THREAD", "MAX_GPU_HISTORIES", "MAX_CUTS_HISTORIES", "MAX_ENDPOINTS_HISTORIES",
@@ -279,6 +279,9 @@ extern "C" int pct_params_derive(pct_params* p, char* err, size_t err_len)
     if (p->hull_type < PCT_HULL_IMPORT || p->hull_type > PCT_HULL_FBP) return fail("HULL_TYPE out of range");
     if (p->x_0_type < PCT_X0_IMPORT || p->x_0_type > PCT_X0_ZEROS) return fail("X_0_TYPE out of range");
     if (p->fbp_filter_type < 0 || p->fbp_filter_type > 2) return fail("FBP_FILTER_TYPE out of range");
+    if (p->mlp_algorithm != PCT_MLP_STANDARD && p->mlp_algorithm != PCT_MLP_TABULATED)
+        return fail("MLP_ALGORITHM out of range");
+    if (p->list_segment_histories < 0) return fail("LIST_SEGMENT_HISTORIES must be >= 0");
     if (p->fbp_med_filter_radius < 0 || p->fbp_med_filter_radius > 10) return fail("FBP_MED_FILTER_RADIUS must be 0..10");
     if (p->hull_avg_filter_radius < 0 || p->hull_avg_filter_radius > 64)
         return fail("HULL_AVG_FILTER_RADIUS must be 0..64");

@@ -372,6 +372,7 @@ extern "C" int pct_import_preprocessed(pct_ctx* ctx, const char* dir)
     ctx->n_hist = ctx->n_recon = n;
     ctx->n_loaded = ctx->n_recon_volume = ctx->n_post_cuts = 0;
     ctx->list_entries = total;
+    ctx->list_slots = total;
     ctx->drop_k = 0;
     ctx->tvs_L = 0;
     ctx->tvs_log.clear();

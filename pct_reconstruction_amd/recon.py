@@ -145,10 +145,22 @@ class Reconstruction:
         """MLP voxel lists of every reconstruction history (find_MLP_path_GPU_tabulated, DS.cu:9104)."""
         self._call("pct_mlp_build_lists")
 
-    def DROP(self, iterations: int | None = None) -> None:
-        """DROP iterations over the lists (DROP_GPU, DS.cu:9848)."""
+    def DROP(self, iterations: int | None = None, write_dir=None) -> None:
+        """DROP iterations over the lists (DROP_GPU, DS.cu:9848).  With write_dir, every iterate is written
+        there as x_<k>.txt/.dat/.png after its iteration (WRITE_X_KI, DS.cu:9915-9922)."""
         its = self.params.iterations if iterations is None else iterations
-        self._call("pct_drop_iterate", int(its))
+        if write_dir is None:
+            self._call("pct_drop_iterate", int(its))
+            return
+        for _ in range(int(its)):
+            self._call("pct_drop_iterate", 1)
+            self.write_reference_layout(write_dir, _lib.WRITE_X)
+
+    def write_reference_layout(self, directory, flags: int = 15) -> None:
+        """FBP.txt, hull.*, x_0.* and x_<k>.* in the reference's output names (pct_write_reference_layout)."""
+        from pathlib import Path
+        Path(directory).mkdir(parents=True, exist_ok=True)
+        self._call("pct_write_reference_layout", str(directory).encode(), int(flags))
 
     def image_reconstruction(self, iterations: int | None = None) -> None:
         """reconstruction_cuts + MLP + DROP (DS.cu:10650)."""

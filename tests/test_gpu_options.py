@@ -57,3 +57,34 @@ def test_parity_mm_input_with_shifts_through_projection_files(oracle, tmp_path):
         files.append(f)
     assert np.array_equal(mm["angle_deg"], np.trunc(mm["angle_deg"]))  # integer angles: PCTD v0 files
     _run_and_compare(P, mm, oracle, "mm-shifts-files", files=files)
+
+
+def test_parity_untabulated_mlp(oracle):
+    """MLP_ALGORITHM = STANDARD (find_MLP_path_GPU, DS.cu:8895-9103): polynomial Sigma terms, log() scattering
+    coefficients and cos()/sin() rotation per history; lists, chords and the DROP image against the oracle."""
+    P, data = small_head(n=40000, seed=29, mlp_algorithm=0)
+    _run_and_compare(P, data, oracle, "head-untabulated-MLP")
+
+
+def test_reference_layout_files(tmp_path):
+    """FBP.txt, hull.*, x_0.* and x_<k>.* after every DROP iteration, in the reference's names and layouts
+    (array_2_disk DS.cu:10743, write_PNG :9939; call sites :6112, :7365, :8637, :9915)."""
+    from pct_reconstruction_amd import Reconstruction, io
+    P, data = small_cylinder(n=20000, seed=5, iterations=2)
+    with Reconstruction(P) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.preprocessing()
+        r.write_reference_layout(tmp_path)
+        r.reconstruction_cuts()
+        r.build_mlp_lists()
+        r.DROP(2, write_dir=tmp_path)
+        x, x0, hull = r.image("x"), r.image("x0"), r.image("hull")
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert names == sorted(["FBP.txt", "hull.txt", "hull.dat", "hull.png", "x_0.txt", "x_0.dat", "x_0.png",
+                            "x_1.txt", "x_1.dat", "x_1.png", "x_2.txt", "x_2.dat", "x_2.png"])
+    io.write_image_text(tmp_path / "check.txt", x)
+    assert (tmp_path / "x_2.txt").read_bytes() == (tmp_path / "check.txt").read_bytes()
+    assert np.array_equal(np.fromfile(tmp_path / "x_0.dat", np.float32), (x0 / 2).reshape(-1))
+    lines = (tmp_path / "hull.txt").read_text().splitlines()
+    assert len(lines) == P.rows * P.slices
+    assert np.array_equal(np.array([ln.split() for ln in lines], float).reshape(-1), hull.reshape(-1))

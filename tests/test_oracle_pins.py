@@ -150,6 +150,76 @@ def test_mlp_steps_match_quadrature_solve(oracle, k):
     assert np.abs(ref[:, 1] - t_line).max() > 0.01 or k == 3
 
 
+def mlp_standard_solve(P, h):
+    """The most-likely path of find_MLP_path_GPU (MLP_ALGORITHM = STANDARD, DS.cu:8895-9103) from the
+    definition, in exact arithmetic: the rotation by the entry angle itself (no trig table), u_1 = k STEP
+    and u_2 unquantised, Sigma_1 / Sigma_2 as quadratures of the scattering integrand over [0, u_1] and
+    [u_1, u_2] times the scattering coefficient at u_1 and at u_2 - u_1, and a numpy.linalg.solve of the
+    normal equations (MLP_Algebraic_Expansion.tex:515-900, :1511 on)."""
+    STEP = P.mlp_u_step
+    xi, yi, zi, xo, yo, zo = (float(h[k]) for k in range(6))
+    xy_e, xz_e, xy_x, xz_x = (float(h[k]) for k in (6, 7, 8, 9))
+    c, s = math.cos(xy_e), math.sin(xy_e)
+    if c * xi + s * yi > c * xo + s * yo:
+        xy_e += math.pi
+        xy_x += math.pi
+        c, s = math.cos(xy_e), math.sin(xy_e)
+    u_in, u_out = c * xi + s * yi, c * xo + s * yo
+    t_in, t_out = c * yi - s * xi, c * yo - s * xo
+    u2 = abs(u_out - u_in)
+    out = []
+    k = 1
+    while k * STEP < u2 - STEP:
+        u1 = k * STEP
+        s1 = [_scat(u1) * _q(g, 0, u1) for g in (lambda x: (u1 - x) ** 2 * _f(x), lambda x: (u1 - x) * _f(x), _f)]
+        s2 = [_scat(u2 - u1) * _q(g, u1, u2) for g in (lambda x: (u2 - x) ** 2 * _f(x), lambda x: (u2 - x) * _f(x),
+                                                       _f)]
+
+        def inverse(m):
+            st, stth, sth = m
+            return np.array([[sth, -stth], [-stth, st]]) / (st * sth - stth * stth)
+        S1i, S2i = inverse(s1), inverse(s2)
+        R1 = np.array([[1.0, u2 - u1], [0.0, 1.0]])
+        M = S1i + R1.T @ S2i @ R1
+        t1 = np.linalg.solve(M, S1i @ np.array([t_in, 0.0]) + R1.T @ S2i @ np.array([t_out, xy_x - xy_e]))[0]
+        v1 = np.linalg.solve(M, S1i @ np.array([zi, 0.0]) + R1.T @ S2i @ np.array([zo, xz_x - xz_e]))[0]
+        out.append((u_in + u1, t1, v1))
+        k += 1
+    return np.array(out), (c, s)
+
+
+@pytest.mark.parametrize("k", range(len(_PROTONS)))
+def test_mlp_standard_steps_match_quadrature_solve(oracle, k):
+    """The untabulated MLP: every step's t_1 and v_1 against the exact solve.  The reference rounds u_1^2,
+    u_1^3, u_2^2, the two scattering coefficients and two determinant squares to float (powf), which moves
+    the path by well under 1e-5 cm; a wrong sign, index or matrix product moves it by millimetres."""
+    P = _params(mlp_algorithm=0)
+    T = oracle.MlpTables(P)
+    h = np.array(_PROTONS[k], np.float32)
+    rec, path, chord = oracle.mlp_steps(P, T, h, 0)
+    ref, (c, s) = mlp_standard_solve(P, h)
+    assert rec.shape[0] == ref.shape[0] > 200
+    np.testing.assert_allclose(rec[:, 0], ref[:, 0], rtol=0, atol=1e-11)
+    assert np.abs(rec[:, 1] - ref[:, 1]).max() < 1e-5
+    assert np.abs(rec[:, 2] - ref[:, 2]).max() < 1e-5
+    np.testing.assert_allclose(rec[:, 3], c * rec[:, 0] - s * rec[:, 1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rec[:, 4], s * rec[:, 0] + c * rec[:, 1], rtol=0, atol=1e-12)
+    # the listed voxels are the first voxel, then each step's voxel where it changes
+    vox = rec[:, 5].astype(np.int64)
+    dedup = [0]
+    for v in vox:
+        if v != dedup[-1]:
+            dedup.append(int(v))
+    assert path.tolist() == dedup
+    # the chord is EffectiveChordLength of the angles before the +pi turn (DS.cu:8921 precedes :8945)
+    assert chord == oracle.chord_length(P, (float(h[6]) + float(h[8])) / 2.0, (float(h[7]) + float(h[9])) / 2.0)
+    # the tabulated path (table-grid angle, u_1 and u_2, tabulated coefficients) is within 1e-3 cm of it
+    Pt = _params()
+    rt, _, _ = oracle.mlp_steps(Pt, oracle.MlpTables(Pt), h, 0)
+    m = min(rt.shape[0], rec.shape[0])
+    assert np.abs(rt[:m, 3:5] - rec[:m, 3:5]).max() < 1e-3
+
+
 def test_mlp_entry_exit_swap_symmetry(oracle):
     """Entry <-> exit swap.  The reference's 1/(beta^2 p^2) polynomial is a function of the depth from the
     entry (the proton slows down), so the tabulated MLP is not symmetric in general.  With a constant
