@@ -129,38 +129,40 @@ def fp64_oracle_error(r, P, iterations: int) -> dict:
             "sample": f"all {chord.size} reconstruction histories, DROP from x0 in FP64"}
 
 
-def fp64_oracle_error_sampled(P, ph, bm, seed: int, n: int, iterations: int, blocks_per_segment: int) -> dict:
-    """The large-200M path on a sample of its workload: `n` histories with the same geometry, beam and DROP blocks,
-    lists built per segment of `blocks_per_segment` blocks, `iterations` DROP iterations from x0, against the
-    oracle's FP64 DROP over the oracle's own lists."""
+def fp64_oracle_error_sampled(P, ph, bm, seed: int, n: int, iterations: int, blocks_per_segment: int, hull,
+                              x0) -> dict:
+    """The large-200M path on a sample of its workload: `n` fresh histories with the same geometry and beam,
+    reconstructed on the full run's hull and x0 (HULL_TYPE = X_0_TYPE = IMPORT), lists built per segment of
+    `blocks_per_segment` DROP blocks, `iterations` DROP iterations, against the oracle's FP64 DROP over the
+    oracle's own lists from the same x0."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pct_oracle as O
     from pct_reconstruction_amd import Reconstruction, synth
     Q = P.copy()
     Q.list_segment_histories = blocks_per_segment * Q.drop_block_size
+    Q.hull_type = 0
+    Q.x_0_type = 0
     data = synth.generate(ph, bm, seed + 1, n)
     with Reconstruction(Q, device=0) as r:
         r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
-        r.preprocessing()
+        r.set_hull(hull)
+        r.preprocessing(x0)
         r.reconstruction_cuts()
         r.build_mlp_lists()
         r.DROP(iterations)
         x = r.image("x").reshape(-1).astype(np.float64)
-        x0 = r.image("x0").reshape(-1).astype(np.float64)
         segments = r.metrics()["counts"].get("segments", {}).get("count", 0)
-        names = ["x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "xy_entry_angle", "xz_entry_angle",
-                 "xy_exit_angle", "xz_exit_angle"]
-        h = np.ascontiguousarray(np.stack([r.histories(k) for k in names]))
-        first = np.ascontiguousarray(r.histories("first_voxel"))
-        w = np.ascontiguousarray(r.histories("wepl"))
-    off, vox, chord = O.mlp_lists(Q, O.MlpTables(Q), h, first)
-    x64 = x0
+        n_recon = r.count("reconstruction")
+    ref = O.run_pipeline(Q, data, stop_after="lists", hull_import=hull, x0_import=x0)
+    assert ref["offsets"].size - 1 == n_recon
+    x64 = np.asarray(x0, np.float64).reshape(-1)
     for _ in range(iterations):
-        x64 = O.drop_iteration_fp64(Q, off, vox, chord, w, x64)
+        x64 = O.drop_iteration_fp64(Q, ref["offsets"], ref["voxels"], ref["chord"], ref["wepl_recon"], x64)
     return {"iterations": iterations, "rel_l2": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
             "max_abs": float(np.abs(x - x64).max()),
-            "sample": f"{n} histories of the same geometry and beam ({chord.size} reconstructed, {segments} list "
-                      f"segments of {blocks_per_segment} blocks), DROP from x0, oracle FP64 DROP on its own lists"}
+            "sample": f"{n} fresh histories of the same geometry and beam on the run's hull and x0 ({n_recon} "
+                      f"reconstructed, {segments} list segments of {blocks_per_segment} blocks), DROP from x0, "
+                      f"oracle FP64 DROP on its own lists"}
 
 
 def main() -> int:
@@ -171,10 +173,10 @@ def main() -> int:
     ap.add_argument("--workload", default="head-50M", choices=["head-50M", "large-200M", "large-200M-segment"],
                     help="head-50M (the benchmark); large-200M: 200M histories on 512x512x128 @ 0.5x0.5x1 mm, "
                          "230 MeV, SM hull, block 120000, 12 iterations, MLP lists and DROP plans built per "
-                         "segment of 115 blocks inside every iteration, learned start; large-200M-segment: one "
+                         "segment of 58 blocks inside every iteration, learned start; large-200M-segment: one "
                          "25M-history share of it with resident lists")
     ap.add_argument("--histories", type=int, default=None, help="default: 50M (head-50M), 200M, 25M (segment)")
-    ap.add_argument("--segment-histories", type=int, default=13_800_000,
+    ap.add_argument("--segment-histories", type=int, default=6_960_000,
                     help="large-200M: reconstruction histories per list segment (whole DROP blocks)")
     ap.add_argument("--iterations", type=int, default=None, help="DROP iterations per step (default: config)")
     ap.add_argument("--cpu-sample", type=int, default=0,
@@ -345,12 +347,13 @@ def main() -> int:
         cpu = cpu_baseline(r.params, r, sample, threads, iters, max(args.cpu_iterations, 1))
 
     stages = {k: round(v, 3) for k, v in m["stage_ms"].items()}
+    run_hull, run_x0 = r.image("hull").astype(np.uint8), r.image("x0")
     if learned is not None:
         learned.close()
     r.close()
     if args.workload == "large-200M" and not args.no_fp64:
         log("FP64 oracle on a 1M-history sample of the workload")
-        fp64 = fp64_oracle_error_sampled(P, ph, bm, args.seed, 1_000_000, iters, 2)
+        fp64 = fp64_oracle_error_sampled(P, ph, bm, args.seed, 1_000_000, iters, 2, run_hull, run_x0)
         log(f"FP64 oracle (sample): rel L2 {fp64['rel_l2']:.3e}")
     out = {
         "metric": "MLP+DROP proton histories/s; time to fixed residual: learned x0 vs FBP x0",

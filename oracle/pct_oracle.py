@@ -67,6 +67,12 @@ def lib() -> C.CDLL:
         "on_num_params": ([], i64),
         "on_loss_grad": ([C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp], C.c_double),
     }
+    twins = ["o_intersections", "o_binning", "o_stats_and_cuts", "o_construct_sinogram", "o_sc", "o_msc", "o_sm",
+             "o_filter", "o_backproject", "o_fbp_hull", "o_median_filter_2d", "o_endpoints", "o_chord_length",
+             "o_mlp_lists", "o_mlp_steps", "o_drop_iteration_fp64", "o_residual", "o_tables_create",
+             "o_tables_destroy"]
+    for name in twins:  # the FP64 twin (pct_oracle_fp64.c): same arguments, double arrays
+        sig[name + "_f64"] = sig[name]
     for name, (a, r) in sig.items():
         fn = getattr(L, name)
         fn.argtypes = a
@@ -89,6 +95,15 @@ def _arr(arrs, n):
     return (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
 
 
+def _fn(name, fp64):
+    """Entry point `name` of the reference-precision oracle, or of its FP64 twin."""
+    return getattr(lib(), name + ("_f64" if fp64 else ""))
+
+
+def _real(fp64):
+    return np.float64 if fp64 else np.float32
+
+
 H_NAMES = ["x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "xy_entry_angle", "xz_entry_angle",
            "xy_exit_angle", "xz_exit_angle"]
 
@@ -99,37 +114,41 @@ def derive(P):
     return q
 
 
-def intersections(P, t, v, u, angle):
+def intersections(P, t, v, u, angle, fp64=False):
     n = angle.shape[0]
-    h = np.zeros((10, n), np.float32)
+    R = _real(fp64)
+    h = np.zeros((10, n), R)
     missed = np.zeros(n, np.uint8)
-    tt = [np.ascontiguousarray(x, np.float32) for x in t]
-    vv = [np.ascontiguousarray(x, np.float32) for x in v]
-    uu = [np.ascontiguousarray(x, np.float32) for x in u]
+    tt = [np.ascontiguousarray(x, R) for x in t]
+    vv = [np.ascontiguousarray(x, R) for x in v]
+    uu = [np.ascontiguousarray(x, R) for x in u]
     ang = np.ascontiguousarray(angle, np.float32)
-    lib().o_intersections(C.byref(P), n, _arr(tt, 4), _arr(vv, 4), _arr(uu, 4), _p(ang), _arr(list(h), 10), _p(missed))
+    _fn("o_intersections", fp64)(C.byref(P), n, _arr(tt, 4), _arr(vv, 4), _arr(uu, 4), _p(ang), _arr(list(h), 10),
+                                 _p(missed))
     return h, missed.astype(bool)
 
 
-def binning(P, h):
+def binning(P, h, fp64=False):
     n = h.shape[1]
     b = np.zeros(n, np.int32)
     r = np.zeros(n, np.uint8)
-    lib().o_binning(C.byref(P), n, _arr(list(h), 10), _p(b), _p(r))
+    _fn("o_binning", fp64)(C.byref(P), n, _arr(list(h), 10), _p(b), _p(r))
     return b, r
 
 
-def stats_and_cuts(P, bin_num, in_range, wepl, h):
+def stats_and_cuts(P, bin_num, in_range, wepl, h, fp64=False):
     n = wepl.shape[0]
     NB = P.num_bins
+    R = _real(fp64)
     counts_pre = np.zeros(NB, np.int32)
-    mean = np.zeros((3, NB), np.float32)
-    std = np.zeros((3, NB), np.float32)
+    mean = np.zeros((3, NB), R)
+    std = np.zeros((3, NB), R)
     counts_post = np.zeros(NB, np.int32)
-    sino = np.zeros(NB, np.float32)
+    sino = np.zeros(NB, R)
     failed = np.zeros(n, np.uint8)
-    lib().o_stats_and_cuts(C.byref(P), n, _p(bin_num), _p(in_range), _p(wepl), _arr(list(h), 10), _p(counts_pre),
-                           _arr(list(mean), 3), _arr(list(std), 3), _p(counts_post), _p(sino), _p(failed))
+    _fn("o_stats_and_cuts", fp64)(C.byref(P), n, _p(bin_num), _p(in_range), _p(wepl), _arr(list(h), 10),
+                                  _p(counts_pre), _arr(list(mean), 3), _arr(list(std), 3), _p(counts_post), _p(sino),
+                                  _p(failed))
     return dict(counts_pre=counts_pre, mean=mean, std=std, counts_post=counts_post, sinogram_sum=sino,
                 failed=failed.astype(bool))
 
@@ -143,14 +162,10 @@ def init_hull(P, dtype):
     return out
 
 
-def carve(P, method, h, wepl):
+def carve(P, method, h, wepl, fp64=False):
     n = wepl.shape[0]
-    if method == "sc":
-        out = init_hull(P, np.uint8)
-        lib().o_sc(C.byref(P), n, _arr(list(h), 10), _p(wepl), _p(out))
-    else:
-        out = init_hull(P, np.int32)
-        getattr(lib(), "o_" + method)(C.byref(P), n, _arr(list(h), 10), _p(wepl), _p(out))
+    out = init_hull(P, np.uint8 if method == "sc" else np.int32)
+    _fn("o_" + method, fp64)(C.byref(P), n, _arr(list(h), 10), _p(wepl), _p(out))
     return out
 
 
@@ -166,33 +181,36 @@ def sm_edge(P, counts):
     return out
 
 
-def construct_sinogram(P, counts_post, sino_sum):
+def construct_sinogram(P, counts_post, sino_sum, fp64=False):
     s = sino_sum.copy()
-    lib().o_construct_sinogram(C.byref(P), _p(counts_post), _p(s))
+    _fn("o_construct_sinogram", fp64)(C.byref(P), _p(counts_post), _p(s))
     return s
 
 
-def filter_sinogram(P, sino):
-    out = np.zeros(P.num_bins, np.float32)
-    lib().o_filter(C.byref(P), _p(np.ascontiguousarray(sino, np.float32)), _p(out))
+def filter_sinogram(P, sino, fp64=False):
+    R = _real(fp64)
+    out = np.zeros(P.num_bins, R)
+    _fn("o_filter", fp64)(C.byref(P), _p(np.ascontiguousarray(sino, R)), _p(out))
     return out
 
 
-def backproject(P, filtered):
-    out = np.zeros(P.num_voxels, np.float32)
-    lib().o_backproject(C.byref(P), _p(np.ascontiguousarray(filtered, np.float32)), _p(out))
+def backproject(P, filtered, fp64=False):
+    R = _real(fp64)
+    out = np.zeros(P.num_voxels, R)
+    _fn("o_backproject", fp64)(C.byref(P), _p(np.ascontiguousarray(filtered, R)), _p(out))
     return out
 
 
-def fbp_hull(P, image):
+def fbp_hull(P, image, fp64=False):
     out = np.zeros(P.num_voxels, np.uint8)
-    lib().o_fbp_hull(C.byref(P), _p(np.ascontiguousarray(image, np.float32)), _p(out))
+    _fn("o_fbp_hull", fp64)(C.byref(P), _p(np.ascontiguousarray(image, _real(fp64))), _p(out))
     return out
 
 
-def median_filter_2d(P, image, radius):
-    out = np.zeros(P.num_voxels, np.float32)
-    lib().o_median_filter_2d(C.byref(P), _p(np.ascontiguousarray(image, np.float32).reshape(-1)), int(radius), _p(out))
+def median_filter_2d(P, image, radius, fp64=False):
+    R = _real(fp64)
+    out = np.zeros(P.num_voxels, R)
+    _fn("o_median_filter_2d", fp64)(C.byref(P), _p(np.ascontiguousarray(image, R).reshape(-1)), int(radius), _p(out))
     return out
 
 
@@ -203,11 +221,12 @@ def avg_filter_hull(P, hull, radius, threshold):
     return out
 
 
-def endpoints(P, hull_u8, h):
-    h = np.ascontiguousarray(h.copy(), np.float32)
+def endpoints(P, hull_u8, h, fp64=False):
+    h = np.ascontiguousarray(h.copy(), _real(fp64))
     n = h.shape[1]
     first = np.zeros(n, np.uint32)
-    lib().o_endpoints(C.byref(P), _p(np.ascontiguousarray(hull_u8, np.uint8)), n, _arr(list(h), 10), _p(first))
+    _fn("o_endpoints", fp64)(C.byref(P), _p(np.ascontiguousarray(hull_u8, np.uint8)), n, _arr(list(h), 10),
+                             _p(first))
     return h, first
 
 
@@ -232,31 +251,31 @@ class MlpTables:
         return np.ctypeslib.as_array(getattr(self.t, name), shape=(n,)).copy()
 
 
-def mlp_lists(P, tables: MlpTables, h, first):
-    h = np.ascontiguousarray(h, np.float32)
+def mlp_lists(P, tables: MlpTables, h, first, fp64=False):
+    h = np.ascontiguousarray(h, _real(fp64))
     n = h.shape[1]
     off = np.zeros(n + 1, np.int64)
     chord = np.zeros(n, np.float64)
     cap = max(16, int(n) * (int(2 * P.recon_cyl_radius / P.mlp_u_step) + 4))
     vox = np.zeros(cap, np.uint32)
-    total = lib().o_mlp_lists(C.byref(P), C.byref(tables.t), n, _arr(list(h), 10),
-                              _p(np.ascontiguousarray(first, np.uint32)), _p(off), _p(vox), cap, _p(chord))
+    total = _fn("o_mlp_lists", fp64)(C.byref(P), C.byref(tables.t), n, _arr(list(h), 10),
+                                     _p(np.ascontiguousarray(first, np.uint32)), _p(off), _p(vox), cap, _p(chord))
     if total < 0:
         raise RuntimeError("oracle MLP capacity")
     return off, vox[:total].copy(), chord
 
 
-def mlp_steps(P, tables: MlpTables, hist10, first):
+def mlp_steps(P, tables: MlpTables, hist10, first, fp64=False):
     """Every u-step of the MLP loop for one history: returns (steps, path, chord) where steps is a
     (k, 6) array of u, t_1, v_1, x_1, y_1, voxel."""
-    h = np.ascontiguousarray(hist10, np.float32).reshape(10)
+    h = np.ascontiguousarray(hist10, _real(fp64)).reshape(10)
     cap = 8 * (int(2 * P.recon_cyl_radius / P.mlp_u_step) + 8)
     rec = np.zeros((cap, 6), np.float64)
     path = np.zeros(cap, np.uint32)
     plen = C.c_int64()
     chord = C.c_double()
-    k = lib().o_mlp_steps(C.byref(P), C.byref(tables.t), _p(h), int(first), _p(rec), cap, _p(path), cap,
-                          C.byref(plen), C.byref(chord))
+    k = _fn("o_mlp_steps", fp64)(C.byref(P), C.byref(tables.t), _p(h), int(first), _p(rec), cap, _p(path), cap,
+                                 C.byref(plen), C.byref(chord))
     if k > cap or plen.value < 0:
         raise RuntimeError("oracle MLP step capacity")
     return rec[:k].copy(), path[:plen.value].copy(), chord.value
@@ -270,9 +289,12 @@ def drop_iteration(P, off, vox, chord, wepl, x, accum=1, nthreads=1):
 
 
 def drop_iteration_fp64(P, off, vox, chord, wepl, x64):
+    """DROP with x, the sums and the scalars in double; WEPL of the float pipeline (float32) or of the FP64
+    pipeline (float64)."""
     x = np.ascontiguousarray(x64, np.float64).copy()
-    lib().o_drop_iteration_fp64(C.byref(P), off.shape[0] - 1, _p(off), _p(vox), _p(chord),
-                                _p(np.ascontiguousarray(wepl, np.float32)), _p(x))
+    fp64 = np.asarray(wepl).dtype == np.float64
+    _fn("o_drop_iteration_fp64", fp64)(C.byref(P), off.shape[0] - 1, _p(off), _p(vox), _p(chord),
+                                       _p(np.ascontiguousarray(wepl, _real(fp64))), _p(x))
     return x
 
 
@@ -387,13 +409,16 @@ def tvs_iteration(P, hull, x, k, L, mode=TVS_SUM_TILED_FP64):
     return x, Lc.value, float(tb.value), float(ta.value)
 
 
-def convert_units(P, t, v, u, wepl):
-    """convert_mm_2_cm (DS.cu:4070) and apply_tuv_shifts (DS.cu:4090) in float as the reference does."""
+def convert_units(P, t, v, u, wepl, fp64=False):
+    """convert_mm_2_cm (DS.cu:4070) and apply_tuv_shifts (DS.cu:4090) in float as the reference does (in double
+    for the FP64 pipeline)."""
+    R = _real(fp64)
+
     def mm(a):
-        return (a.astype(np.float64) * 0.1).astype(np.float32) if P.data_in_mm else a
+        return (a.astype(np.float64) * 0.1).astype(R) if P.data_in_mm else a.astype(R)
 
     def sh(a, s):
-        return (a.astype(np.float64) + s).astype(np.float32) if (P.t_shift or P.u_shift or P.v_shift) else a
+        return (a.astype(np.float64) + s).astype(R) if (P.t_shift or P.u_shift or P.v_shift) else a
     t = sh(mm(np.asarray(t, np.float32)), P.t_shift)
     u = sh(mm(np.asarray(u, np.float32)), P.u_shift)
     v = sh(mm(np.asarray(v, np.float32)), P.v_shift)
@@ -401,14 +426,21 @@ def convert_units(P, t, v, u, wepl):
 
 
 def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=False, tvs_sum=TVS_SUM_TILED_FP64,
-                 hull_import=None, x0_import=None):
+                 hull_import=None, x0_import=None, fp64=False):
     """The reference pipeline on the CPU.  Returns a dict with every intermediate result.  hull_import and
-    x0_import are the images HULL_TYPE = IMPORT and X_0_TYPE = IMPORT read (DS.cu:7358, :8614)."""
+    x0_import are the images HULL_TYPE = IMPORT and X_0_TYPE = IMPORT read (DS.cu:7358, :8614).
+
+    fp64=True runs the FP64 twin of every stage (pct_oracle_fp64.c): per-history geometry, bin statistics,
+    sinogram, FBP, x0 and the DROP iterate in double, the float rounding points of the reference taken
+    exactly; the integer decisions (bins, cuts, hulls, endpoints, lists) follow from those values."""
     P = derive(P)
     out = {}
-    t, v, u, w = convert_units(P, data["t"], data["v"], data["u"], data["wepl"])
-    h, missed = intersections(P, t, v, u, data["angle_deg"])
-    bin_num, in_range = binning(P, h)
+    R = _real(fp64)
+    if fp64 and P.tvs_on:
+        raise ValueError("the FP64 pipeline has no TVS twin")
+    t, v, u, w = convert_units(P, data["t"], data["v"], data["u"], data["wepl"], fp64)
+    h, missed = intersections(P, t, v, u, data["angle_deg"], fp64)
+    bin_num, in_range = binning(P, h, fp64)
     keep = ~missed
     ids = np.nonzero(keep)[0]  # loaded-history index of each surviving history
     h = np.ascontiguousarray(h[:, keep])
@@ -417,16 +449,16 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     in_range = np.ascontiguousarray(in_range[keep])
     out.update(h_volume=h, wepl_volume=w, bin_num=bin_num, in_range=in_range, missed=missed)
     if P.sc_on:
-        out["sc_hull"] = carve(P, "sc", h, w)
+        out["sc_hull"] = carve(P, "sc", h, w, fp64)
     if P.msc_on:
-        out["msc_counts"] = carve(P, "msc", h, w)
+        out["msc_counts"] = carve(P, "msc", h, w, fp64)
         out["msc_hull"] = msc_edge(P, out["msc_counts"])
     if P.sm_on:
-        out["sm_counts"] = carve(P, "sm", h, w)
+        out["sm_counts"] = carve(P, "sm", h, w, fp64)
         out["sm_hull"] = sm_edge(P, out["sm_counts"])
     if stop_after == "hulls":
         return out
-    st = stats_and_cuts(P, bin_num, in_range, w, h)
+    st = stats_and_cuts(P, bin_num, in_range, w, h, fp64)
     out.update(st)
     keep = ~st["failed"]
     ids = ids[keep]
@@ -436,16 +468,16 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     out.update(h_cut=h, wepl_cut=w)
     if stop_after == "cuts":
         return out
-    sino = construct_sinogram(P, st["counts_post"], st["sinogram_sum"])
+    sino = construct_sinogram(P, st["counts_post"], st["sinogram_sum"], fp64)
     out["sinogram"] = sino
     if P.fbp_on:
-        out["sinogram_filtered"] = filter_sinogram(P, sino)
-        out["fbp"] = backproject(P, out["sinogram_filtered"])
+        out["sinogram_filtered"] = filter_sinogram(P, sino, fp64)
+        out["fbp"] = backproject(P, out["sinogram_filtered"], fp64)
         fbp_x0 = out["fbp"]
         if P.median_filter_fbp and P.fbp_med_filter_radius > 0 and not P.avg_filter_fbp:  # DS.cu:6150
-            fbp_x0 = out["fbp_median"] = median_filter_2d(P, out["fbp"], P.fbp_med_filter_radius)
+            fbp_x0 = out["fbp_median"] = median_filter_2d(P, out["fbp"], P.fbp_med_filter_radius, fp64)
         if P.hull_type == 4:
-            out["fbp_hull"] = fbp_hull(P, out["fbp"])
+            out["fbp_hull"] = fbp_hull(P, out["fbp"], fp64)
     hull = {0: lambda: np.asarray(hull_import).reshape(-1) != 0,
             1: lambda: out["sc_hull"], 2: lambda: (out["msc_hull"] == 1).astype(np.uint8),
             3: lambda: (out["sm_hull"] == 1).astype(np.uint8), 4: lambda: out["fbp_hull"]}[P.hull_type]()
@@ -456,19 +488,19 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
         hull_ep = out["hull_filtered"] = avg_filter_hull(P, hull, P.hull_avg_filter_radius,
                                                          P.hull_avg_filter_threshold)
     if P.x_0_type == 0:
-        x0 = np.ascontiguousarray(x0_import, np.float32).reshape(-1).copy()
+        x0 = np.ascontiguousarray(x0_import, R).reshape(-1).copy()
     elif P.x_0_type == 1:
-        x0 = hull.astype(np.float32)
+        x0 = hull.astype(R)
     elif P.x_0_type == 2:
         x0 = fbp_x0.copy()
     elif P.x_0_type == 3:
-        x0 = np.where(hull != 0, fbp_x0, np.float32(0)).astype(np.float32)
+        x0 = np.where(hull != 0, fbp_x0, R(0)).astype(R)
     else:
-        x0 = np.zeros(P.num_voxels, np.float32)
+        x0 = np.zeros(P.num_voxels, R)
     out["x0"] = x0
     if stop_after == "fbp":
         return out
-    h2, first = endpoints(P, hull_ep, h)
+    h2, first = endpoints(P, hull_ep, h, fp64)
     keep = first != P.num_voxels
     ids = ids[keep]
     out.update(bin_recon=np.ascontiguousarray(bin_cut[keep]),
@@ -480,7 +512,7 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     if stop_after == "endpoints":
         return out
     tabs = MlpTables(P)
-    off, vox, chord = mlp_lists(P, tabs, h2, first)
+    off, vox, chord = mlp_lists(P, tabs, h2, first, fp64)
     out.update(offsets=off, voxels=vox, chord=chord)
     if stop_after == "lists":
         return out
@@ -490,6 +522,10 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     xs = []
     L, tvs_log = 0, []
     for k in range(1, its + 1):  # DROP_GPU, DS.cu:9885-9915
+        if fp64:
+            x = drop_iteration_fp64(P, off, vox, chord, w2, x)
+            xs.append(x.copy())
+            continue
         if P.tvs_on and P.tvs_first:
             x, L, tb, ta = tvs_iteration(P, hull, x, k, L, tvs_sum)
             tvs_log.append((tb, ta, L))

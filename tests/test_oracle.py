@@ -186,6 +186,19 @@ def test_stats_and_cuts_brute_force(oracle):
         assert st["std"][0][b] == sd
     assert st["failed"][::50].sum() > 0.8 * len(st["failed"][::50])   # the +8 cm outliers are cut
     assert st["counts_post"].sum() == (~st["failed"]).sum()
+    # the FP64 twin: double sums in history order, sample standard deviation, and the same cut decisions
+    # wherever a history is not within 1e-6 sigma of a cut boundary
+    s64 = oracle.stats_and_cuts(P, bins, in_range, w.astype(np.float64), h.astype(np.float64), fp64=True)
+    for b in (0, 1, 7, 39):
+        sel = np.nonzero(bins == b)[0]
+        ins = sel[in_range[sel] == 1]
+        m = sum(float(w[i]) for i in ins) / len(ins)
+        assert abs(s64["mean"][0][b] - m) <= 1e-12 * abs(m)
+        sd = math.sqrt(sum((float(w[i]) - m) ** 2 for i in sel) / (len(ins) - 1))
+        assert abs(s64["std"][0][b] - sd) <= 1e-12 * sd
+    margin = np.abs(np.abs(w - s64["mean"][0][bins]) - 3 * s64["std"][0][bins]) / s64["std"][0][bins]
+    clear = margin > 1e-6
+    assert (s64["failed"][clear] == st["failed"][clear]).all()
 
 
 # ---------------------------------------------------------------- hulls
@@ -385,6 +398,8 @@ def test_filter_impulse_response(oracle):
     scale = 260.7 / math.sqrt(260.7 ** 2 + t * t + v * v)
     expect = 0.5 * 1.0 / ((0.5 * math.pi) ** 2 * (1 - 4 * sep * sep)) * scale
     np.testing.assert_allclose(out, expect.astype(np.float32), rtol=1e-6)
+    out64 = oracle.filter_sinogram(P, sino.astype(np.float64), fp64=True)[:T]
+    np.testing.assert_allclose(out64, expect, rtol=1e-13)
 
 
 def test_fbp_reconstructs_water_rsp(oracle):

@@ -144,6 +144,10 @@ def test_mlp_steps_match_quadrature_solve(oracle, k):
     # reference's float rounding
     exact, _ = mlp_reference_solve(P, h, float_squares=False)
     assert np.abs(rec[:, 1:3] - exact[:, 1:3]).max() < 1e-5
+    # the FP64 twin takes those squares exactly: it agrees with the exact solve to rounding
+    r64, _, _ = oracle.mlp_steps(P, T, h.astype(np.float64), 0, fp64=True)
+    assert r64.shape[0] == exact.shape[0]
+    assert np.abs(r64[:, 1:3] - exact[:, 1:3]).max() < 1e-10
     # the case is not degenerate: the path leaves the straight chord between entry and exit
     u0, u1 = ref[0, 0] - P.mlp_u_step, ref[0, 0] + (ref.shape[0]) * P.mlp_u_step
     t_line = np.interp(ref[:, 0], [u0, u1], [c * h[1] - s * h[0], c * h[4] - s * h[3]])
@@ -199,6 +203,9 @@ def test_mlp_standard_steps_match_quadrature_solve(oracle, k):
     rec, path, chord = oracle.mlp_steps(P, T, h, 0)
     ref, (c, s) = mlp_standard_solve(P, h)
     assert rec.shape[0] == ref.shape[0] > 200
+    # the FP64 twin has none of the float roundings: it agrees with the exact solve to rounding
+    r64, _, _ = oracle.mlp_steps(P, T, h.astype(np.float64), 0, fp64=True)
+    assert np.abs(r64[:, 1:3] - ref[:, 1:3]).max() < 1e-10
     np.testing.assert_allclose(rec[:, 0], ref[:, 0], rtol=0, atol=1e-11)
     assert np.abs(rec[:, 1] - ref[:, 1]).max() < 1e-5
     assert np.abs(rec[:, 2] - ref[:, 2]).max() < 1e-5
@@ -318,6 +325,10 @@ def test_intersections_angled_protons(oracle):
     h, missed = oracle.intersections(P, t, v, u, ang)
     assert not missed.any()
     T64, V64, U64 = t.astype(np.float64), v.astype(np.float64), u.astype(np.float64)
+    # the FP64 twin: no float rounding of the discriminant, slopes or stored coordinates
+    h64, missed64 = oracle.intersections(P, T64, V64, U64, ang, fp64=True)
+    assert not missed64.any()
+    tol = {False: (2e-5, 2e-6), True: (1e-9, 1e-12)}
     for i in range(n):
         rot = math.radians(float(ang[i]))
         cr, sr = math.cos(rot), math.sin(rot)
@@ -327,15 +338,17 @@ def test_intersections_angled_protons(oracle):
                               (U64[2, i], T64[2, i]))
         xe, ye = cr * ue - sr * te, sr * ue + cr * te
         xx, yx = cr * ux - sr * tx, sr * ux + cr * tx
-        assert abs(h[0, i] - xe) < 2e-5 and abs(h[1, i] - ye) < 2e-5, i
-        assert abs(h[3, i] - xx) < 2e-5 and abs(h[4, i] - yx) < 2e-5, i
         ze = V64[1, i] + (V64[1, i] - V64[0, i]) / (U64[1, i] - U64[0, i]) * (ue - U64[1, i])
         zx = V64[2, i] + (V64[3, i] - V64[2, i]) / (U64[3, i] - U64[2, i]) * (ux - U64[2, i])
-        assert abs(h[2, i] - ze) < 2e-5 and abs(h[5, i] - zx) < 2e-5, i
-        assert abs(h[6, i] - (math.atan2(T64[1, i] - T64[0, i], 5.0) + rot)) < 2e-6
-        assert abs(h[8, i] - (math.atan2(T64[3, i] - T64[2, i], 5.0) + rot)) < 2e-6
-        assert abs(h[7, i] - math.atan2(V64[1, i] - V64[0, i], 5.0)) < 2e-6
-        assert abs(h[9, i] - math.atan2(V64[3, i] - V64[2, i], 5.0)) < 2e-6
+        for hh, fp64 in ((h, False), (h64, True)):
+            tp, ta = tol[fp64]
+            assert abs(hh[0, i] - xe) < tp and abs(hh[1, i] - ye) < tp, (i, fp64)
+            assert abs(hh[3, i] - xx) < tp and abs(hh[4, i] - yx) < tp, (i, fp64)
+            assert abs(hh[2, i] - ze) < tp and abs(hh[5, i] - zx) < tp, (i, fp64)
+            assert abs(hh[6, i] - (math.atan2(T64[1, i] - T64[0, i], 5.0) + rot)) < ta
+            assert abs(hh[8, i] - (math.atan2(T64[3, i] - T64[2, i], 5.0) + rot)) < ta
+            assert abs(hh[7, i] - math.atan2(V64[1, i] - V64[0, i], 5.0)) < ta
+            assert abs(hh[9, i] - math.atan2(V64[3, i] - V64[2, i], 5.0)) < ta
 
 
 def test_intersections_steep_line_and_exit_through_the_top(oracle):
@@ -515,6 +528,10 @@ def test_backprojection_of_linear_sinogram_closed_form(oracle, narrow):
     assert (img[~inside] == np.float32(P.rsp_air)).all()
     err = np.abs(img[inside] - exp[inside]) / np.maximum(np.abs(exp[inside]), 1.0)
     assert err.max() < 1e-5       # float accumulation over the angle bins; a dropped or mis-signed term is O(1e-2)
+    # the FP64 twin accumulates in double: the closed form to rounding
+    img64 = oracle.backproject(P, sino3.reshape(-1).astype(np.float64), fp64=True).reshape(img.shape)
+    err64 = np.abs(img64[inside] - exp[inside]) / np.maximum(np.abs(exp[inside]), 1.0)
+    assert err64.max() < 1e-12
     # the narrow detector really takes the one-sided taps
     assert (sum(edge) > 1000) if narrow else True
 
