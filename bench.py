@@ -66,6 +66,28 @@ def cpu_baseline(P, r, sample: int, threads: int, iterations: int, cpu_iteration
                       f"rate = n*{iterations}/(t_mlp + {iterations}*t_drop)"}
 
 
+def cpu_stage_rates(P, ph, bm, seed: int, n: int, threads: int) -> dict:
+    """Every stage of the oracle's reference-precision pipeline (a)-(e) timed on `n` fresh histories of the
+    workload on the host cores: wall time and histories/s of the histories each stage processes, one DROP
+    iteration with the CPU baseline's float atomic sums."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    import pct_oracle as O
+    from pct_reconstruction_amd import synth
+    data = synth.generate(ph, bm, seed + 2, n)
+    tm = {}
+    out = O.run_pipeline(P, data, iterations=1, accum=0, timings=tm, nthreads=threads)
+    volume, cut, recon = out["h_volume"].shape[1], out["h_cut"].shape[1], out["offsets"].size - 1
+    work = {"intersections": n, "binning": volume, "hulls": volume, "stats_and_cuts": volume, "fbp": None,
+            "hull_selection_x0": None, "endpoints": cut, "mlp_lists": recon, "drop": recon}
+    stages = {}
+    for k, w in work.items():
+        sec = tm.get(k, 0.0)
+        stages[k] = {"ms": 1e3 * sec, "histories": w, "histories_per_s": (w / sec) if (w and sec > 0) else None}
+    return {"sample": f"{n} fresh histories of the workload (seed {seed + 2}): {volume} in the volume, {cut} after "
+                      f"the cuts, {recon} reconstructed", "cores": threads, "stages": stages}
+
+
 def time_to_residual(r, P, weights, ref_iterations: int) -> dict:
     """FBP start: `ref_iterations` DROP iterations give the residual target ||Ax - b||.  Learned start:
     one DROP iteration, the net on that iterate (kept only if it lowers the residual), then DROP until
@@ -165,13 +187,53 @@ def fp64_oracle_error_sampled(P, ph, bm, seed: int, n: int, iterations: int, blo
                       f"oracle FP64 DROP on its own lists"}
 
 
+def fp64_pipeline_error(P, ph, bm, seed: int, n: int, iterations: int) -> dict:
+    """Every image of the library's pipeline on `n` fresh histories of the workload against the FP64 oracle of
+    the whole pipeline (pct_oracle_fp64.c: every stage in double), and against the reference-precision
+    oracle.  Decisions (cuts, hull voxels, reconstructed histories) may differ between float and double
+    arithmetic; the FP64 DROP on the reference-precision lists from the same x0 separates the DROP's own
+    arithmetic error from them."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pct_oracle as O
+    from pct_reconstruction_amd import Reconstruction, synth
+    data = synth.generate(ph, bm, seed + 1, n)
+    with Reconstruction(P, device=0) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.preprocessing()
+        r.image_reconstruction(iterations)
+        img = {k: r.image(k).reshape(-1).astype(np.float64) for k in ("fbp", "x0", "x")}
+        hull = r.image("hull").reshape(-1)
+        counts = {k: r.count(k) for k in ("post_cuts", "reconstruction")}
+    t0 = time.perf_counter()
+    o64 = O.run_pipeline(P, data, iterations=iterations, fp64=True)
+    o32 = O.run_pipeline(P, data, iterations=iterations, accum=P.drop_accum)
+    t_oracle = time.perf_counter() - t0
+    x64 = o32["x0"].astype(np.float64)
+    for _ in range(iterations):
+        x64 = O.drop_iteration_fp64(P, o32["offsets"], o32["voxels"], o32["chord"], o32["wepl_recon"], x64)
+
+    def err(a, b):
+        b = np.asarray(b, np.float64)
+        return {"rel_l2": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)),
+                "max_abs": float(np.abs(a - b).max())}
+    return {
+        "histories": n, "iterations": iterations, "oracle_s": t_oracle,
+        "vs_fp64_pipeline": {k: err(img[k], o64[k]) for k in img},
+        "vs_reference_precision_oracle": {k: err(img[k], o32[k]) for k in img},
+        "drop_vs_fp64_same_lists": err(img["x"], x64),
+        "decisions": {"post_cuts": [counts["post_cuts"], int(o64["h_cut"].shape[1])],
+                      "reconstruction": [counts["reconstruction"], int(o64["offsets"].size - 1)],
+                      "hull_voxels_differing": int(np.sum(hull != o64["hull"]))},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="head-50M", choices=["head-50M", "large-200M", "large-200M-segment"],
-                    help="head-50M (the benchmark); large-200M: 200M histories on 512x512x128 @ 0.5x0.5x1 mm, "
+    ap.add_argument("--workload", default="head-50M", choices=["head-50M", "cylinder-2M", "large-200M", "large-200M-segment"],
+                    help="head-50M (the benchmark); cylinder-2M: the CPU-runnable case; large-200M: 200M histories on 512x512x128 @ 0.5x0.5x1 mm, "
                          "230 MeV, SM hull, block 120000, 12 iterations, MLP lists and DROP plans built per "
                          "segment of 58 blocks inside every iteration, learned start; large-200M-segment: one "
                          "25M-history share of it with resident lists")
@@ -182,9 +244,14 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="reconstruction histories timed on the CPU (0: all of them)")
     ap.add_argument("--cpu-iterations", type=int, default=2, help="DROP iterations timed on the CPU")
+    ap.add_argument("--cpu-stages", type=int, default=2_000_000,
+                    help="fresh histories on which every oracle stage is timed on the CPU (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the time-to-residual measurement")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle comparison")
+    ap.add_argument("--fp64-pipeline", type=int, default=0, metavar="N",
+                    help="also compare every image with the FP64 oracle of the whole pipeline on N fresh histories "
+                         "of the workload (minutes of CPU time for N = 2M)")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -197,6 +264,10 @@ def main() -> int:
     if args.workload == "head-50M":
         P, ph, bm, n = configs.head_50m(args.histories or 50_000_000)
         workload = "head-50M: 256x256x96 @ ~1 mm, MSC+SM hulls, 3-sigma cuts, FBP x0"
+    elif args.workload == "cylinder-2M":
+        P, ph, bm, n = configs.cylinder_2m(args.histories or 2_000_000)
+        workload = ("cylinder-2M: water cylinder + 3 inserts, 200x200x32 @ 1x1x2.5 mm, 90 x 4 deg, SC hull, "
+                    "Shepp-Logan FBP x0")
     elif args.workload == "large-200M":
         P, ph, bm, n = configs.large_200m(args.histories or 200_000_000)
         P.list_segment_histories = args.segment_histories
@@ -345,12 +416,27 @@ def main() -> int:
         sample = n_recon if want <= 0 else min(want, n_recon)
         log(f"cpu baseline on {sample} histories, {threads} threads")
         cpu = cpu_baseline(r.params, r, sample, threads, iters, max(args.cpu_iterations, 1))
+        if args.cpu_stages > 0:
+            log(f"cpu stage rates on {args.cpu_stages} histories")
+            cpu["per_stage"] = cpu_stage_rates(r.params, ph, bm, args.seed, args.cpu_stages, threads)
 
     stages = {k: round(v, 3) for k, v in m["stage_ms"].items()}
+    cnt = m["counts"]
+    per_stage_work = {"intersect_bin_carve": cnt["loaded"], "stats_and_cuts": cnt["recon_volume"],
+                      "mlp_endpoints": cnt["post_cuts"], "mlp_build_lists": n_recon,
+                      # the last DROP call of the timed step: all iterations, or those after the net's start
+                      "drop": n_recon * (iters - 1 if learned is not None else iters)}
+    stage_rates = {k: (w / (stages[k] * 1e-3) if stages.get(k) else None) for k, w in per_stage_work.items()}
     run_hull, run_x0 = r.image("hull").astype(np.uint8), r.image("x0")
     if learned is not None:
         learned.close()
     r.close()
+    fp64_pipe = None
+    if args.fp64_pipeline > 0:
+        log(f"FP64 oracle of the whole pipeline on {args.fp64_pipeline} histories")
+        fp64_pipe = fp64_pipeline_error(P, ph, bm, args.seed, args.fp64_pipeline, iters)
+        log(f"FP64 pipeline: x rel L2 {fp64_pipe['vs_fp64_pipeline']['x']['rel_l2']:.3e}, same lists "
+            f"{fp64_pipe['drop_vs_fp64_same_lists']['rel_l2']:.3e}")
     if args.workload == "large-200M" and not args.no_fp64:
         log("FP64 oracle on a 1M-history sample of the workload")
         fp64 = fp64_oracle_error_sampled(P, ph, bm, args.seed, 1_000_000, iters, 2, run_hull, run_x0)
@@ -367,18 +453,22 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 (MLP, accumulators) / f32 (image)",
-        "data": "synthetic (seeded 10-ellipsoid head phantom, Highland scatter, WEPL noise 2.5 mm, 1% outliers)",
+        "data": ("synthetic (seeded water cylinder with 3 inserts, Highland scatter, WEPL noise 3 mm, 1% outliers)"
+                 if args.workload == "cylinder-2M" else
+                 "synthetic (seeded 10-ellipsoid head phantom, Highland scatter, WEPL noise 2.5 mm, 1% outliers)"),
         "config": {"workload": f"{workload}, MLP + DROP {iters} iterations, block {P.drop_block_size}",
                    "histories_loaded": n, "histories_reconstructed": n_recon, "list_entries": entries,
                    "drop_iterations": iters, "drop_block_size": P.drop_block_size},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_oracle": fp64,
+        "fp64_pipeline": fp64_pipe,
         "hbm_step": hbm_step,
         "time_to_residual": warm,
         "learned_start": judges[-1] if judges else None,
         "kernels": kernels,
         "stages_ms": stages,
+        "stage_histories_per_s": stage_rates,
         "preprocessing_s": t_pre,
         "counts": m["counts"],
     }

@@ -1181,10 +1181,18 @@ void O(o_drop_iteration)(const pct_params* p, int64_t n, const int64_t* offsets,
     float* K32 = (float*)calloc((size_t)NV, sizeof(float)); /* Kahan compensation */
     double* U64 = (double*)calloc((size_t)NV, sizeof(double));
     uint32_t* S = (uint32_t*)calloc((size_t)NV, sizeof(uint32_t));
+    /* per-thread sums, counts and touched-voxel lists of the parallel CPU baseline */
+    float* priv_u = NULL;
+    uint32_t *priv_s = NULL, *priv_t = NULL;
+    if (nthreads < -1) {
+        priv_u = (float*)calloc((size_t)(-nthreads) * NV, sizeof(float));
+        priv_s = (uint32_t*)calloc((size_t)(-nthreads) * NV, sizeof(uint32_t));
+        priv_t = (uint32_t*)malloc((size_t)(-nthreads) * NV * sizeof(uint32_t));
+    }
     for (int64_t start = 0; start < n; start += p->drop_block_size) {                   /* :9549-9582 */
         int64_t proc;
         o_drop_block_range(p, n, start, &proc);
-        if (nthreads <= 1) {
+        if (nthreads >= -1 && nthreads <= 1) {
             for (int64_t i = start; i < start + proc; i++) {                             /* :9321-9380 */
                 int64_t cnt = offsets[i + 1] - offsets[i];
                 const uint32_t* a = voxels + offsets[i];
@@ -1210,7 +1218,9 @@ void O(o_drop_iteration)(const pct_params* p, int64_t n, const int64_t* offsets,
             }
         } else {
 #ifdef _OPENMP
-            /* CPU baseline: histories of a block in parallel, float atomic sums (order varies) */
+          if (nthreads > 1) {
+            /* CPU baseline: the histories of a block in parallel, float atomic sums (order varies, as the
+             * reference's float atomicAdd) */
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
             for (int64_t i = start; i < start + proc; i++) {
                 int64_t cnt = offsets[i + 1] - offsets[i];
@@ -1231,7 +1241,49 @@ void O(o_drop_iteration)(const pct_params* p, int64_t n, const int64_t* offsets,
                     }
                 }
             }
-#pragma omp parallel for num_threads(nthreads) schedule(static)
+          } else {
+            /* the same with per-thread sums (nthreads < -1): each thread sums its histories' updates and
+             * counts into its own x_update / S with a list of the voxels it touched, and adds them into the
+             * block's at the block end */
+            const int nth = -nthreads;
+#pragma omp parallel num_threads(nth)
+            {
+                const int tid = omp_get_thread_num();
+                float* Ut = priv_u + (size_t)tid * NV;
+                uint32_t* St = priv_s + (size_t)tid * NV;
+                uint32_t* Tt = priv_t + (size_t)tid * NV;
+                int64_t nt = 0;
+#pragma omp for schedule(dynamic, 64)
+                for (int64_t i = start; i < start + proc; i++) {
+                    int64_t cnt = offsets[i + 1] - offsets[i];
+                    const uint32_t* a = voxels + offsets[i];
+                    double L = o_row_chord(chord, i);
+                    double ax = 0.0;
+                    for (int64_t j = 0; j < cnt; j++) ax += x[a[j]];
+                    ax *= L;
+                    double aa = (double)f_sq(L) * cnt;
+                    if (cnt >= p->min_mlp_length) {
+                        double val = L * ((wepl[i] - ax) / aa) * lambda;
+                        float fv = (float)val;
+                        for (int64_t j = 0; j < cnt; j++) {
+                            uint32_t v = a[j];
+                            if (St[v]++ == 0) Tt[nt++] = v;
+                            Ut[v] += fv;
+                        }
+                    }
+                }
+                for (int64_t k = 0; k < nt; k++) {   /* the block-end reduction of this thread's sums */
+                    uint32_t v = Tt[k];
+#pragma omp atomic
+                    S[v] += St[v];
+#pragma omp atomic
+                    U32[v] += Ut[v];
+                    St[v] = 0;
+                    Ut[v] = 0.0f;
+                }
+            }
+          }
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : -nthreads) schedule(static)
             for (int64_t v = 0; v < NV; v++) {
                 if (S[v] > 0) {
                     x[v] += U32[v] / (float)S[v];
@@ -1257,6 +1309,9 @@ void O(o_drop_iteration)(const pct_params* p, int64_t n, const int64_t* offsets,
     free(U32);
     free(U64);
     free(S);
+    free(priv_u);
+    free(priv_s);
+    free(priv_t);
 }
 
 void O(o_drop_iteration_fp64)(const pct_params* p, int64_t n, const int64_t* offsets, const uint32_t* voxels,

@@ -107,8 +107,9 @@ int64_t o_mlp_steps(const pct_params* p, const o_tables* t, const float* hist10,
 /* One DROP iteration over precomputed lists (DROP_full_tx_iteration :9537, calculate_x_update_GPU
  * :9321, DROP_block_update_GPU :9522, image_update_GPU :9398) in history order.
  * accum = PCT_ACCUM_FP32: x_update in float (reference); PCT_ACCUM_FP64: x_update in double.
- * nthreads > 1: OpenMP over histories of a block, per-thread sums reduced at the block end (CPU
- * baseline timing; summation order then differs). */
+ * nthreads > 1: OpenMP over histories of a block with float atomic sums (CPU baseline timing; the
+ * summation order then varies); nthreads < -1: |nthreads| threads with per-thread sums, counts and
+ * touched-voxel lists added into the block's at the block end. */
 void o_drop_iteration(const pct_params* p, int64_t n, const int64_t* offsets, const uint32_t* voxels,
                       const double* chord, const float* wepl, float* x, int accum, int nthreads);
 /* The same iteration with the image and every sum in double (the FP64 oracle). */

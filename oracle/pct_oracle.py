@@ -426,21 +426,34 @@ def convert_units(P, t, v, u, wepl, fp64=False):
 
 
 def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=False, tvs_sum=TVS_SUM_TILED_FP64,
-                 hull_import=None, x0_import=None, fp64=False):
+                 hull_import=None, x0_import=None, fp64=False, timings=None, nthreads=1):
     """The reference pipeline on the CPU.  Returns a dict with every intermediate result.  hull_import and
     x0_import are the images HULL_TYPE = IMPORT and X_0_TYPE = IMPORT read (DS.cu:7358, :8614).
 
     fp64=True runs the FP64 twin of every stage (pct_oracle_fp64.c): per-history geometry, bin statistics,
     sinogram, FBP, x0 and the DROP iterate in double, the float rounding points of the reference taken
-    exactly; the integer decisions (bins, cuts, hulls, endpoints, lists) follow from those values."""
+    exactly; the integer decisions (bins, cuts, hulls, endpoints, lists) follow from those values.
+
+    timings: a dict that receives the wall time of each stage in seconds; nthreads: OpenMP threads of the DROP
+    iterations (the CPU baseline's float atomic sums, order-dependent; 1 = history order)."""
+    import time
     P = derive(P)
     out = {}
+    clock = [time.perf_counter()]
+
+    def lap(name):
+        t = time.perf_counter()
+        if timings is not None:
+            timings[name] = timings.get(name, 0.0) + (t - clock[0])
+        clock[0] = t
     R = _real(fp64)
     if fp64 and P.tvs_on:
         raise ValueError("the FP64 pipeline has no TVS twin")
     t, v, u, w = convert_units(P, data["t"], data["v"], data["u"], data["wepl"], fp64)
     h, missed = intersections(P, t, v, u, data["angle_deg"], fp64)
+    lap("intersections")
     bin_num, in_range = binning(P, h, fp64)
+    lap("binning")
     keep = ~missed
     ids = np.nonzero(keep)[0]  # loaded-history index of each surviving history
     h = np.ascontiguousarray(h[:, keep])
@@ -456,9 +469,11 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     if P.sm_on:
         out["sm_counts"] = carve(P, "sm", h, w, fp64)
         out["sm_hull"] = sm_edge(P, out["sm_counts"])
+    lap("hulls")
     if stop_after == "hulls":
         return out
     st = stats_and_cuts(P, bin_num, in_range, w, h, fp64)
+    lap("stats_and_cuts")
     out.update(st)
     keep = ~st["failed"]
     ids = ids[keep]
@@ -478,6 +493,7 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
             fbp_x0 = out["fbp_median"] = median_filter_2d(P, out["fbp"], P.fbp_med_filter_radius, fp64)
         if P.hull_type == 4:
             out["fbp_hull"] = fbp_hull(P, out["fbp"], fp64)
+    lap("fbp")
     hull = {0: lambda: np.asarray(hull_import).reshape(-1) != 0,
             1: lambda: out["sc_hull"], 2: lambda: (out["msc_hull"] == 1).astype(np.uint8),
             3: lambda: (out["sm_hull"] == 1).astype(np.uint8), 4: lambda: out["fbp_hull"]}[P.hull_type]()
@@ -500,7 +516,9 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
     out["x0"] = x0
     if stop_after == "fbp":
         return out
+    lap("hull_selection_x0")
     h2, first = endpoints(P, hull_ep, h, fp64)
+    lap("endpoints")
     keep = first != P.num_voxels
     ids = ids[keep]
     out.update(bin_recon=np.ascontiguousarray(bin_cut[keep]),
@@ -513,6 +531,7 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
         return out
     tabs = MlpTables(P)
     off, vox, chord = mlp_lists(P, tabs, h2, first, fp64)
+    lap("mlp_lists")
     out.update(offsets=off, voxels=vox, chord=chord)
     if stop_after == "lists":
         return out
@@ -529,7 +548,9 @@ def run_pipeline(P, data, iterations=None, stop_after=None, accum=1, with_fp64=F
         if P.tvs_on and P.tvs_first:
             x, L, tb, ta = tvs_iteration(P, hull, x, k, L, tvs_sum)
             tvs_log.append((tb, ta, L))
-        x = drop_iteration(P, off, vox, chord, w2, x, accum=accum)
+        lap("tvs")
+        x = drop_iteration(P, off, vox, chord, w2, x, accum=accum, nthreads=nthreads)
+        lap("drop")
         if P.tvs_on and not P.tvs_first:
             x, L, tb, ta = tvs_iteration(P, hull, x, k, L, tvs_sum)
             tvs_log.append((tb, ta, L))
