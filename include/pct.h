@@ -49,13 +49,16 @@ enum pct_drop_accum { PCT_ACCUM_FP32 = 0, PCT_ACCUM_FP64 = 1, PCT_ACCUM_KAHAN32 
  * voxels that stages their entries' updates in LDS with coalesced loads; COLS_SLICED stores the
  * ranks of each wavefront's 64 voxels column-major (entry m of the 64 voxels contiguous, padded to
  * the largest count), so each rank load of the per-voxel sums is one coalesced read.  COLS_LDS takes
- * precedence over COLS_SLICED. */
+ * precedence over COLS_SLICED.  BUCKETS (with BRICKS) transposes the lists by tiles of 4 x 4 bricks, each
+ * sorted in LDS, instead of one device-wide radix sort per group of blocks; a group with a tile of more
+ * than 8192 entries in one block takes the sort. */
 enum pct_drop_flags {
     PCT_DROP_SPATIAL_ORDER = 1,
     PCT_DROP_XCD_SLABS = 2,
     PCT_DROP_BRICKS = 4,
     PCT_DROP_COLS_LDS = 8,
-    PCT_DROP_COLS_SLICED = 16
+    PCT_DROP_COLS_SLICED = 16,
+    PCT_DROP_BUCKETS = 32
 };
 
 /*

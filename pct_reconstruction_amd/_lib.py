@@ -23,7 +23,7 @@ X0_IMPORT, X0_HULL, X0_FBP, X0_HYBRID, X0_ZEROS = 0, 1, 2, 3, 4
 MLP_STANDARD, MLP_TABULATED = 0, 1
 WRITE_FBP, WRITE_HULL, WRITE_X0, WRITE_X = 1, 2, 4, 8
 ACCUM_FP32, ACCUM_FP64, ACCUM_KAHAN32 = 0, 1, 2
-DROP_SPATIAL_ORDER, DROP_XCD_SLABS, DROP_BRICKS, DROP_COLS_LDS, DROP_COLS_SLICED = 1, 2, 4, 8, 16
+DROP_SPATIAL_ORDER, DROP_XCD_SLABS, DROP_BRICKS, DROP_COLS_LDS, DROP_COLS_SLICED, DROP_BUCKETS = 1, 2, 4, 8, 16, 32
 DROP_DEFAULT_FLAGS = DROP_SPATIAL_ORDER | DROP_BRICKS | DROP_COLS_SLICED  # the context's default (internal.h)
 
 IMG = {"x": 0, "fbp": 1, "x0": 2, "hull": 3, "sc_hull": 4, "msc_counts": 5, "sm_counts": 6, "msc_hull": 7,

@@ -115,7 +115,7 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
     }
     if (const char* e = std::getenv("PCT_DROP_FLAGS")) {
         int f = std::atoi(e);
-        if (f >= 0 && f <= 31) ctx->drop_flags = f;
+        if (f >= 0 && f <= 63) ctx->drop_flags = f;
     }
     if (const char* e = std::getenv("PCT_DROP_GROUP_ENTRIES")) {
         long long g = std::atoll(e);
@@ -568,6 +568,9 @@ const char* pct_metrics_json(pct_ctx* ctx)
     }
     std::snprintf(buf, sizeof(buf), "}, \"kernel_ms\": {\"mlp_lists\": %.4f}", ctx->t_mlp_kernel);
     s += buf;
+    std::snprintf(buf, sizeof(buf), ", \"drop_plan\": {\"flags\": %d, \"groups_bucketed\": %lld, \"groups_sorted\": %lld}",
+                  ctx->plan.flags, (long long)ctx->plan.groups_bucketed, (long long)ctx->plan.groups_sorted);
+    s += buf;
     double tvs_ms = 0;
     for (const TvsRecord& r : ctx->tvs_log) tvs_ms += r.ms;
     std::snprintf(buf, sizeof(buf), ", \"tvs\": {\"iterations\": %zu, \"ms\": %.4f, \"L\": %d}}", ctx->tvs_log.size(),
@@ -644,8 +647,8 @@ extern "C" int pct_set_launch(pct_ctx* ctx, int drop_row_lanes, int drop_col_vox
     if (!ctx) return PCT_E_ARG;
     const int g = drop_row_lanes, v = drop_col_voxels;
     if (!(g == 4 || g == 8 || g == 16 || g == 32 || g == 64) || !(v == 1 || v == 2 || v == 4 || v == 8)) return PCT_E_ARG;
-    if (drop_flags &
-        ~(PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS | PCT_DROP_BRICKS | PCT_DROP_COLS_LDS | PCT_DROP_COLS_SLICED))
+    if (drop_flags & ~(PCT_DROP_SPATIAL_ORDER | PCT_DROP_XCD_SLABS | PCT_DROP_BRICKS | PCT_DROP_COLS_LDS |
+                       PCT_DROP_COLS_SLICED | PCT_DROP_BUCKETS))
         return PCT_E_ARG;
     ctx->drop_row_lanes = g;
     ctx->drop_col_voxels = v;

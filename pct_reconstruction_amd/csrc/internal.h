@@ -112,6 +112,12 @@ struct DropPlan {
     int nbx = 0, nby = 0;
     int64_t cap_entries = 0, cap_off = 0, cap_r = 0, cap_perm = 0, cap_group_e = 0, cap_group_h = 0,
             cap_group_b = 0;
+    // bucket transposition (PCT_DROP_BUCKETS): per bucket counts, offsets, cursors, runs, run offsets; size
+    // class lists (3 x buckets) and counters (3 + overflow flag)
+    uint32_t *bk_cnt = nullptr, *bk_off = nullptr, *bk_cur = nullptr, *bk_runs = nullptr, *bk_roff = nullptr;
+    uint32_t *bk_list = nullptr, *bk_misc = nullptr;
+    int64_t cap_bk = 0;
+    int64_t groups_bucketed = 0, groups_sorted = 0;  // groups transposed by each path in the last build
     int flags = -1;              // PCT_DROP_* flags the plan was built with
     bool desc_all = false;       // desc holds a descriptor for every reconstruction history
     bool ready = false;

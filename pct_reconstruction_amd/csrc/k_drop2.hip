@@ -190,6 +190,246 @@ __global__ void k_group_runs(const uint32_t* __restrict__ run_keys, const uint32
     if (lb < n_blocks && (u == 0 || (run_keys[u - 1] >> vbits) != lb)) run_start[lb] = (int32_t)u;
 }
 
+// ---- bucket transposition (once per list build; PCT_DROP_BUCKETS) ----
+// The same transposed lists as the sort path, built without a device-wide sort.  A bucket is a tile of
+// 4 x 4 bricks (16 x 16 x 2 voxels, 512 voxel slots) of one block.  (1) Every history's entries are cut
+// into runs of one tile and counted per bucket (one atomic per run); (2) after a scan the runs are
+// scattered into their buckets (unordered between histories); (3) one workgroup per bucket sorts it in
+// LDS by (voxel slot in the tile, history index in the block) with 8-bit LSD passes, writes the ranks in
+// that order and the bucket's runs of equal voxels; (4) the runs are compacted in bucket order.  Within a
+// voxel the entries end in history order, as in the sort path, so every column sum is formed in the same
+// order; only the order of the runs differs (tile-major instead of brick-major), which no sum depends on.
+constexpr int kTileBits = 9;  // 16 bricks x 32 voxels
+
+struct TileGeom {
+    uint32_t nbx, nby, ntbx, ntby;  // bricks and 4 x 4-brick tiles per row / column
+    uint32_t nt;                     // tiles per block
+};
+__device__ __forceinline__ uint32_t tile_of(uint32_t v, TileGeom t)
+{
+    const uint32_t b = v >> 5, bx = b % t.nbx, r = b / t.nbx, by = r % t.nby, bz = r / t.nby;
+    return (bz * t.ntby + (by >> 2)) * t.ntbx + (bx >> 2);
+}
+__device__ __forceinline__ uint32_t slot_of(uint32_t v, TileGeom t)
+{
+    const uint32_t b = v >> 5, bx = b % t.nbx, by = (b / t.nbx) % t.nby;
+    return (((by & 3u) << 2) | (bx & 3u)) << 5 | (v & 31u);
+}
+__device__ __forceinline__ uint32_t voxel_of(uint32_t tile, uint32_t slot, TileGeom t)
+{
+    const uint32_t tbx = tile % t.ntbx, r = tile / t.ntbx, tby = r % t.ntby, bz = r / t.ntby;
+    const uint32_t by = (tby << 2) | (slot >> 7), bx = (tbx << 2) | ((slot >> 5) & 3u);
+    return ((bz * t.nby + by) * t.nbx + bx) << 5 | (slot & 31u);
+}
+
+// Lane l of the 16 lanes of a history takes the contiguous entries [l c / 16, (l + 1) c / 16) and cuts
+// them into runs of one bucket.  COUNT: one atomic add of the run length per run.  Otherwise: the run's
+// place in its bucket from one atomic on the bucket's cursor, and its (slot << hb | history index, rank)
+// pairs written there.
+template <bool COUNT>
+__global__ void k_tr_runs(int64_t h0, int64_t n, int64_t BS, int64_t rb0, TileGeom tg, int hb,
+                          const int64_t* __restrict__ off, const uint32_t* __restrict__ vox,
+                          const uint32_t* __restrict__ long_cnt, const uint32_t* __restrict__ rank,
+                          uint32_t* __restrict__ bucket, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    constexpr int L = 16;
+    const int lane = threadIdx.x & (L - 1);
+    const int64_t n_groups = ((int64_t)gridDim.x * blockDim.x) / L;
+    for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L; t < n; t += n_groups) {
+        const uint32_t c = long_cnt[t];
+        if (c == 0) continue;
+        const int64_t i = h0 + t, rb = i / BS;
+        const uint32_t j0 = (uint32_t)(((uint64_t)c * lane) / L), j1 = (uint32_t)(((uint64_t)c * (lane + 1)) / L);
+        if (j0 >= j1) continue;
+        const uint32_t* lst = vox + off[i];
+        const uint32_t gbase = (uint32_t)(rb - rb0) * tg.nt;
+        const uint32_t hi = (uint32_t)(i - rb * BS), rk = COUNT ? 0u : rank[i];
+        uint32_t j = j0;
+        while (j < j1) {
+            const uint32_t g = gbase + tile_of(lst[j], tg);
+            uint32_t e = j + 1;
+            while (e < j1 && gbase + tile_of(lst[e], tg) == g) e++;
+            if (COUNT) {
+                atomicAdd(&bucket[g], e - j);
+            } else {
+                const uint32_t p = atomicAdd(&bucket[g], e - j);
+                for (uint32_t q = j; q < e; q++) {
+                    keys[p + q - j] = (slot_of(lst[q], tg) << hb) | hi;
+                    vals[p + q - j] = rk;
+                }
+            }
+            j = e;
+        }
+    }
+}
+
+// Buckets by size: class 0 up to 1024 entries, 1 up to 4096, 2 up to 8192; larger buckets raise *over.
+__global__ void k_tr_classify(uint32_t nbk, const uint32_t* __restrict__ cnt, uint32_t* __restrict__ lists,
+                              uint32_t* __restrict__ n_in, uint32_t* __restrict__ over)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nbk) return;
+    const uint32_t c = cnt[g];
+    if (c == 0) return;
+    const int k = c <= 1024u ? 0 : c <= 4096u ? 1 : c <= 8192u ? 2 : 3;
+    if (k == 3) {
+        atomicOr(over, 1u);
+        return;
+    }
+    lists[(size_t)k * nbk + atomicAdd(&n_in[k], 1u)] = g;
+}
+
+// One workgroup of W waves per bucket of at most CAP entries: LSD radix sort of the (key, rank) pairs in LDS
+// with 8-bit digits over `bits` key bits (stable: wave w owns a contiguous quarter of the items and ranks
+// its lanes in order), then the ranks in sorted order to t_hist and the bucket's runs of equal slots.
+template <int W, int CAP>
+__global__ void __launch_bounds__(W * 64) k_tr_sort(const uint32_t* __restrict__ list, int bits, int hb,
+                                                   uint32_t nt, TileGeom tg,
+                                                   const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bcnt,
+                                                   const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ t_hist, uint32_t* __restrict__ r_vox,
+                                                   uint32_t* __restrict__ r_off, uint32_t* __restrict__ b_runs)
+{
+    constexpr int T = W * 64;
+    __shared__ uint32_t ka[CAP], va[CAP], kb[CAP], vb[CAP];
+    __shared__ uint32_t cur[W][256];
+    __shared__ uint32_t tot[256];
+    __shared__ uint32_t wsum[W];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t g = list[blockIdx.x];
+    const uint32_t n = bcnt[g], base = boff[g];
+    for (uint32_t j = threadIdx.x; j < n; j += T) {
+        ka[j] = keys[base + j];
+        va[j] = vals[base + j];
+    }
+    const uint32_t s0 = (uint32_t)(((uint64_t)n * w) / W), s1 = (uint32_t)(((uint64_t)n * (w + 1)) / W);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t *src_k = ka, *src_v = va, *dst_k = kb, *dst_v = vb;
+    for (int shift = 0; shift < bits; shift += 8) {
+        for (int d = threadIdx.x; d < W * 256; d += T) cur[d / 256][d % 256] = 0u;
+        __syncthreads();
+        for (uint32_t j = s0 + lane; j < s1; j += 64) atomicAdd(&cur[w][(src_k[j] >> shift) & 255u], 1u);
+        __syncthreads();
+        // cursor of (wave w, digit d) = items of smaller digits + items of digit d in waves before w
+        for (int d = threadIdx.x; d < 256; d += T) {
+            uint32_t s = 0;
+            for (int q = 0; q < W; q++) s += cur[q][d];
+            tot[d] = s;
+        }
+        __syncthreads();
+        if (w == 0) {
+            uint32_t a[4], run = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                a[q] = tot[lane * 4 + q];
+                run += a[q];
+            }
+            uint32_t incl = run;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            uint32_t ex = incl - run;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t d = lane * 4 + q;
+                uint32_t c = ex;
+                for (int p = 0; p < W; p++) {
+                    const uint32_t h = cur[p][d];
+                    cur[p][d] = c;
+                    c += h;
+                }
+                ex += a[q];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j0 = s0; j0 < s1; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            const bool ok = j < s1;
+            const uint32_t k = ok ? src_k[j] : 0u;
+            const uint32_t d = (k >> shift) & 255u;
+            uint64_t m = __ballot(ok);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const uint64_t bb = __ballot(ok && ((d >> b) & 1u));
+                m &= ((d >> b) & 1u) ? bb : ~bb;
+            }
+            if (ok) {
+                const uint32_t p = cur[w][d] + (uint32_t)__popcll(m & lt);
+                dst_k[p] = k;
+                dst_v[p] = src_v[j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (ok && (m & lt) == 0ull) cur[w][d] += (uint32_t)__popcll(m);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        uint32_t* tk = src_k; src_k = dst_k; dst_k = tk;
+        uint32_t* tv = src_v; src_v = dst_v; dst_v = tv;
+    }
+    // ranks in sorted order; runs of equal slots (a run starts at item 0 and wherever the slot changes)
+    const uint32_t tile = g % nt;
+    uint32_t runs_before = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += T) {
+        const uint32_t j = r0 + threadIdx.x;
+        const bool ok = j < n;
+        bool head = false;
+        uint32_t slot = 0;
+        if (ok) {
+            t_hist[base + j] = src_v[j];
+            slot = src_k[j] >> hb;
+            head = (j == 0) || ((src_k[j - 1] >> hb) != slot);
+        }
+        const uint64_t hm = __ballot(head);
+        if (lane == 0) wsum[w] = (uint32_t)__popcll(hm);
+        __syncthreads();
+        uint32_t before = runs_before;
+        for (int q = 0; q < w; q++) before += wsum[q];
+        uint32_t all = runs_before;
+        for (int q = 0; q < W; q++) all += wsum[q];
+        if (head) {
+            const uint32_t u = before + (uint32_t)__popcll(hm & lt);
+            r_vox[base + u] = voxel_of(tile, slot, tg);
+            r_off[base + u] = base + j;
+        }
+        runs_before = all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) b_runs[g] = runs_before;
+}
+
+// Runs of every bucket, in bucket order, into the group's t_vox / t_off (one wavefront per bucket), and
+// the total entries as the closing offset.
+__global__ void k_tr_compact(uint32_t nbk, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ b_runs,
+                             const uint32_t* __restrict__ b_roff, const uint32_t* __restrict__ r_vox,
+                             const uint32_t* __restrict__ r_off, uint32_t* __restrict__ t_vox,
+                             uint32_t* __restrict__ t_off)
+{
+    const uint32_t g = (uint32_t)(((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (g > nbk) return;
+    if (g == nbk) {
+        if (lane == 0) t_off[b_roff[nbk]] = boff[nbk];
+        return;
+    }
+    const uint32_t nr = b_runs[g], src = boff[g], dst = b_roff[g];
+    for (uint32_t k = lane; k < nr; k += 64) {
+        t_vox[dst + k] = r_vox[src + k];
+        t_off[dst + k] = r_off[src + k];
+    }
+}
+
+// first run of every block of the group (-1 when it has none)
+__global__ void k_tr_run_start(uint32_t n_blocks, uint32_t nt, const uint32_t* __restrict__ b_roff,
+                               int32_t* __restrict__ run_start)
+{
+    const uint32_t lb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lb >= n_blocks) return;
+    const uint32_t a = b_roff[lb * nt], b = b_roff[(lb + 1) * nt];
+    run_start[lb] = b > a ? (int32_t)a : -1;
+}
+
 // ---- sliced columns (once per list build, after the transposition) ----
 // Slice s of all blocks' slices belongs to the block k with sb[k].slice_base <= s < sb[k + 1].slice_base
 // (sb holds nb + 1 entries, the last one the total).
@@ -652,6 +892,14 @@ void pct_drop_plan_free(DropPlan* D)
     pct_free(&D->slice_w);
     pct_free(&D->slice_blocks);
     D->cap_sell = D->cap_slices = D->cap_slice_w = D->cap_slice_blocks = 0;
+    pct_free(&D->bk_cnt);
+    pct_free(&D->bk_off);
+    pct_free(&D->bk_cur);
+    pct_free(&D->bk_runs);
+    pct_free(&D->bk_roff);
+    pct_free(&D->bk_list);
+    pct_free(&D->bk_misc);
+    D->cap_bk = 0;
     pct_free(&D->vox_b);
     pct_free(&D->x_b);
     D->cap_vox_b = D->cap_x_b = 0;
@@ -659,6 +907,71 @@ void pct_drop_plan_free(DropPlan* D)
     D->cap_entries = D->cap_off = D->cap_r = D->cap_perm = D->cap_group_e = D->cap_group_h = D->cap_group_b = 0;
     D->ready = false;
 }
+
+namespace {
+
+// The bucket transposition of one group (see k_tr_runs).  *done is false when a bucket holds more than 8192
+// entries; the caller then transposes the group with the sort.
+int transpose_group_buckets(pct_ctx* ctx, DropPlan& D, const DropPlan::Group& g, int64_t gi, int64_t run_off,
+                            int64_t BS, const TileGeom& tg, const uint32_t* lvox, uint32_t* n_runs_out, bool* done)
+{
+    hipStream_t s = ctx->stream;
+    const int64_t h0 = g.hist_begin, H = g.hist_end - g.hist_begin, E = g.entries, rb0 = h0 / BS;
+    const uint32_t nbk = (uint32_t)(g.n_blocks * (int64_t)tg.nt);
+    const int hb = bits_for(BS);
+    *done = false;
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(D.bk_cnt, 0, (size_t)(nbk + 1) * sizeof(uint32_t), s));
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(D.bk_misc, 0, 4 * sizeof(uint32_t), s));
+    int64_t wg = (H * 16 + kBlock - 1) / kBlock;
+    if (wg > 65536) wg = 65536;
+    hipLaunchKernelGGL(k_tr_runs<true>, dim3((unsigned)wg), dim3(kBlock), 0, s, h0, H, BS, rb0, tg, hb, ctx->list_off,
+                       lvox, D.lc, D.rank, D.bk_cnt, (uint32_t*)nullptr, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_tr_classify, dim3(grid_for(nbk)), dim3(kBlock), 0, s, nbk, D.bk_cnt, D.bk_list, D.bk_misc,
+                       D.bk_misc + 3);
+    uint32_t misc[4] = {0, 0, 0, 0};
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(misc, D.bk_misc, sizeof(misc), hipMemcpyDeviceToHost, s));
+    size_t tb = ctx->cub_tmp_bytes;
+    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.bk_cnt, D.bk_off, (int)(nbk + 1), s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    if (misc[3]) return PCT_OK;  // a bucket beyond 8192 entries: the sort path
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(D.bk_cur, D.bk_off, (size_t)nbk * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_tr_runs<false>, dim3((unsigned)wg), dim3(kBlock), 0, s, h0, H, BS, rb0, tg, hb, ctx->list_off,
+                       lvox, D.lc, D.rank, D.bk_cur, D.keys, D.vals);
+    const int bits = kTileBits + hb;
+    uint32_t* t_hist = D.t_hist + g.entry_base;
+    PCT_CHECK_HIP(ctx, hipMemsetAsync(D.bk_runs, 0, (size_t)(nbk + 1) * sizeof(uint32_t), s));
+    if (misc[0])
+        hipLaunchKernelGGL((k_tr_sort<1, 1024>), dim3(misc[0]), dim3(64), 0, s, D.bk_list, bits, hb, tg.nt, tg,
+                           D.bk_off, D.bk_cnt, D.keys, D.vals, t_hist, D.keys_sorted, D.runs, D.bk_runs);
+    if (misc[1])
+        hipLaunchKernelGGL((k_tr_sort<4, 4096>), dim3(misc[1]), dim3(256), 0, s, D.bk_list + nbk, bits, hb, tg.nt, tg,
+                           D.bk_off, D.bk_cnt, D.keys, D.vals, t_hist, D.keys_sorted, D.runs, D.bk_runs);
+    if (misc[2])
+        hipLaunchKernelGGL((k_tr_sort<8, 8192>), dim3(misc[2]), dim3(512), 0, s, D.bk_list + 2 * (size_t)nbk, bits, hb,
+                           tg.nt, tg, D.bk_off, D.bk_cnt, D.keys, D.vals, t_hist, D.keys_sorted, D.runs, D.bk_runs);
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    tb = ctx->cub_tmp_bytes;
+    PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.bk_runs, D.bk_roff, (int)(nbk + 1), s));
+    const int64_t cw = ((int64_t)nbk + 1) * 64;
+    hipLaunchKernelGGL(k_tr_compact, dim3((unsigned)((cw + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nbk, D.bk_off,
+                       D.bk_runs, D.bk_roff, D.keys_sorted, D.runs, D.t_vox + run_off + gi, D.t_off + run_off + gi);
+    hipLaunchKernelGGL(k_tr_run_start, dim3(grid_for(g.n_blocks)), dim3(kBlock), 0, s, (uint32_t)g.n_blocks, tg.nt,
+                       D.bk_roff, D.run_start);
+    PCT_CHECK_HIP(ctx, hipGetLastError());
+    uint32_t nr = 0, total = 0;
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&nr, D.bk_roff + nbk, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipMemcpyAsync(&total, D.bk_off + nbk, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    if ((int64_t)total != E || nr < 1 || (int64_t)nr > E) {
+        ctx->err = "DROP bucket transposition: entry or run count out of range in a group";
+        return PCT_E_LIMIT;
+    }
+    *n_runs_out = nr;
+    *done = true;
+    return PCT_OK;
+}
+
+}  // namespace
 
 // Build the visiting order and the transposed lists of every block (once per list build).  Blocks
 // are processed in groups of up to 2^30 list entries: per group one radix sort of the histories by
@@ -811,9 +1124,39 @@ int pct_drop_prepare(pct_ctx* ctx)
         D.vox_b_ready = true;
         lvox = D.vox_b;
     }
+    // bucket transposition: tiles of 4 x 4 bricks per block
+    const bool buckets = bricks && (ctx->drop_flags & PCT_DROP_BUCKETS) != 0;
+    TileGeom tg{};
+    if (buckets) {
+        tg.nbx = (uint32_t)D.nbx;
+        tg.nby = (uint32_t)D.nby;
+        tg.ntbx = (tg.nbx + 3) / 4;
+        tg.ntby = (tg.nby + 3) / 4;
+        tg.nt = tg.ntbx * tg.ntby * (uint32_t)((P.slices + 1) / 2);
+        const int64_t nbk = max_gb * (int64_t)tg.nt + 1;
+        if (nbk >= (1ll << 31)) {
+            ctx->err = "DROP bucket transposition: too many buckets in a group";
+            return PCT_E_LIMIT;
+        }
+        if (D.cap_bk < nbk || !D.bk_cnt) {
+            PCT_TRY(pct_alloc(ctx, &D.bk_cnt, nbk));
+            PCT_TRY(pct_alloc(ctx, &D.bk_off, nbk));
+            PCT_TRY(pct_alloc(ctx, &D.bk_cur, nbk));
+            PCT_TRY(pct_alloc(ctx, &D.bk_runs, nbk));
+            PCT_TRY(pct_alloc(ctx, &D.bk_roff, nbk));
+            PCT_TRY(pct_alloc(ctx, &D.bk_list, 3 * nbk));
+            PCT_TRY(pct_alloc(ctx, &D.bk_misc, 4));
+            D.cap_bk = nbk;
+        }
+    }
+    D.groups_bucketed = D.groups_sorted = 0;
     // library scratch size for the largest group
     {
         size_t need = 0, t = 0;
+        if (buckets) {
+            PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, t, D.bk_cnt, D.bk_off, (int)D.cap_bk, s));
+            need = std::max(need, t);
+        }
         const int E1 = (int)(max_ge + 1), H1 = (int)(max_gh + 1);
         PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t, D.hkey, D.hkey_sorted, D.hidx, D.hidx_sorted,
                                                               H1, 0, 32, s));
@@ -851,9 +1194,17 @@ int pct_drop_prepare(pct_ctx* ctx)
                            ctx->list_cnt, ctx->chord, ctx->wepl, D.perm, D.rank, D.desc);
         group_run_off[gi] = run_off;
         PCT_CHECK_HIP(ctx, hipMemsetAsync(D.run_start, 0xff, (g.n_blocks + 1) * sizeof(int32_t), s));
+        bool bucketed = false;
         if (E > 0) {
             hipLaunchKernelGGL(k_group_long_counts, dim3(grid_for(H)), dim3(kBlock), 0, s, h0, H, BS, proc_dev,
                                ctx->list_cnt, P.min_mlp_length, D.lc);
+            if (buckets)
+                PCT_TRY(transpose_group_buckets(ctx, D, g, gi, run_off, BS, tg, lvox, &n_runs_h[gi], &bucketed));
+        }
+        if (bucketed) {
+            D.groups_bucketed++;
+        } else if (E > 0) {
+            D.groups_sorted++;
             tb = ctx->cub_tmp_bytes;
             PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, tb, D.lc, D.pos, (int)H, s));
             int64_t wg = (H * 16 + kBlock - 1) / kBlock;  // 16 lanes per history

@@ -178,12 +178,12 @@ class Reconstruction:
 
     def set_launch(self, drop_row_lanes: int = 16, drop_col_voxels: int = 1, spatial_order: bool = True,
                    xcd_slabs: bool = False, bricks: bool = True, cols_lds: bool = False,
-                   cols_sliced: bool = True) -> None:
+                   cols_sliced: bool = True, buckets: bool = False) -> None:
         """Launch shape and scheduling of the DROP kernels (pct_set_launch); results do not change.
         The defaults are the library's."""
         flags = ((_lib.DROP_SPATIAL_ORDER if spatial_order else 0) | (_lib.DROP_XCD_SLABS if xcd_slabs else 0)
                  | (_lib.DROP_BRICKS if bricks else 0) | (_lib.DROP_COLS_LDS if cols_lds else 0)
-                 | (_lib.DROP_COLS_SLICED if cols_sliced else 0))
+                 | (_lib.DROP_COLS_SLICED if cols_sliced else 0) | (_lib.DROP_BUCKETS if buckets else 0))
         self._call("pct_set_launch", int(drop_row_lanes), int(drop_col_voxels), flags)
 
     def reset_iterate(self) -> None:

@@ -114,11 +114,16 @@ def _run_and_compare(P, data, oracle, case, hull_import=None, x0_import=None, fi
                       (16, 1, True, False, True, True, False), (64, 4, True, True, False, False, False),
                       (4, 4, True, True, True, True, False), (16, 1, True, False, True, False, False),
                       (16, 2, True, True, True, False, True), (16, 8, False, False, False, False, True),
-                      (16, 4, True, False, True, False, True), (16, 1, True, False, True, True, True)]:
+                      (16, 4, True, False, True, False, True), (16, 1, True, False, True, True, True),
+                      (16, 1, True, False, True, False, True, True), (16, 2, True, True, True, False, False, True),
+                      (16, 1, True, False, True, True, False, True), (16, 1, False, False, True, False, True, True)]:
             r.set_launch(*shape)
             r.reset_iterate()
             r.DROP(P.iterations)
             xs = r.image("x").reshape(-1)
+            if len(shape) > 7 and shape[7]:  # the bucket transposition took every group
+                plan = r.metrics()["drop_plan"]
+                assert plan["groups_bucketed"] > 0 and plan["groups_sorted"] == 0, plan
             if shape[0] == 16:
                 assert _mismatch(xs, x) == 0, shape
             else:
@@ -167,22 +172,27 @@ def test_fbp_median_filter_radii(oracle, radius):
 
 def test_drop_transposition_groups_do_not_change_bits(monkeypatch):
     """Blocks are transposed in groups of up to 2^30 entries; forcing many small groups (one to a few
-    blocks each, including the quirk-A3 block tails) must give the same image bit for bit."""
+    blocks each, including the quirk-A3 block tails) must give the same image bit for bit, with the sort
+    and with the bucket transposition."""
     from pct_reconstruction_amd import Reconstruction
     P, data = small_head(n=60000, seed=5, drop_block_size=5000, histories_per_block=640, iterations=3)
     images = []
-    for group_entries in (None, "150000", "1"):
+    for group_entries, buckets in ((None, False), ("150000", False), ("1", False), ("150000", True), ("1", True)):
         if group_entries is None:
             monkeypatch.delenv("PCT_DROP_GROUP_ENTRIES", raising=False)
         else:
             monkeypatch.setenv("PCT_DROP_GROUP_ENTRIES", group_entries)
         with Reconstruction(P) as r:
             r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+            if buckets:
+                r.set_launch(buckets=True)
             r.run()
             images.append(r.image("x").reshape(-1))
             x0 = r.image("x0").reshape(-1)
-    assert _mismatch(images[1], images[0]) == 0
-    assert _mismatch(images[2], images[0]) == 0
+            if buckets:
+                assert r.metrics()["drop_plan"]["groups_sorted"] == 0
+    for k in range(1, len(images)):
+        assert _mismatch(images[k], images[0]) == 0, k
     assert np.linalg.norm(images[0] - x0) > 0  # DROP changed the image
 
 
@@ -214,7 +224,7 @@ def test_tiny_and_empty_drop_blocks(oracle):
         rel = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
         assert rel <= 1e-6, rel
         for launch in (dict(cols_sliced=False), dict(cols_lds=True), dict(drop_col_voxels=2),
-                       dict(xcd_slabs=True), dict(spatial_order=False, bricks=False)):
+                       dict(xcd_slabs=True), dict(spatial_order=False, bricks=False), dict(buckets=True)):
             r.set_launch(**launch)
             r.reset_iterate()
             r.DROP(P.iterations)

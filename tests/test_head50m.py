@@ -6,8 +6,9 @@ iterations, block 60000), where the whole oracle pipeline would take too long on
     the library's own endpoints, bit for bit;
   - list invariants: counts sum to the entries, every index is a voxel, every list starts at its
     entry voxel;
-  - DROP: two runs from x0 give the same bits, the visiting order and image layout change no bit,
-    and the data residual falls with every iteration.
+  - DROP: two runs from x0 give the same bits, the visiting order, image layout and transposition
+    (sort or buckets) change no bit, and the data residual falls with every iteration;
+  - two DROP iterations within 1e-4 relative L2 of the oracle's FP64 DROP on the same lists.
 """
 from __future__ import annotations
 
@@ -74,10 +75,25 @@ def test_head_50m_full_size_parity_and_invariants(oracle):
         r.reset_iterate()
         r.DROP(3)
         assert np.array_equal(r.image("x").reshape(-1), x3)
-        for launch in (dict(spatial_order=False, bricks=False), dict(cols_sliced=False)):
+        for launch in (dict(spatial_order=False, bricks=False), dict(cols_sliced=False), dict(buckets=True)):
             r.set_launch(**launch)
             r.reset_iterate()
             r.DROP(3)
             assert np.array_equal(r.image("x").reshape(-1), x3), launch
+            if launch.get("buckets"):
+                assert r.metrics()["drop_plan"]["groups_sorted"] == 0
         print(f"[head-50M] DROP residuals {[round(x, 1) for x in res]}; reruns and schedules bit-identical",
               flush=True)
+        # the image after two iterations against the oracle's FP64 DROP over the same lists (x, sums and
+        # scalars in double): the FP64-oracle error of the benchmark's DROP
+        r.set_launch()
+        r.reset_iterate()
+        r.DROP(2)
+        x2 = r.image("x").reshape(-1).astype(np.float64)
+        x64 = r.image("x0").reshape(-1).astype(np.float64)
+        w = r.histories("wepl")
+        for _ in range(2):
+            x64 = oracle.drop_iteration_fp64(P, off, vox, chord, w, x64)
+        rel = float(np.linalg.norm(x2 - x64) / np.linalg.norm(x64))
+        print(f"[head-50M] two DROP iterations vs the FP64 oracle: rel L2 {rel:.3e}", flush=True)
+        assert rel <= 1e-4

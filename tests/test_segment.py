@@ -78,7 +78,7 @@ def test_large_segment_full_size_parity_and_invariants(oracle):
         r.reset_iterate()
         r.DROP(2)
         assert np.array_equal(r.image("x").reshape(-1), x2)
-        for launch in (dict(cols_sliced=False), dict(spatial_order=False, bricks=False)):
+        for launch in (dict(cols_sliced=False), dict(spatial_order=False, bricks=False), dict(buckets=True)):
             r.set_launch(**launch)
             r.reset_iterate()
             r.DROP(2)
