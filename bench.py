@@ -88,6 +88,103 @@ def cpu_stage_rates(P, ph, bm, seed: int, n: int, threads: int) -> dict:
                       f"the cuts, {recon} reconstructed", "cores": threads, "stages": stages}
 
 
+def stages_around(n: int, seed: int, files: int = 10) -> dict:
+    """Host wall time of the stages on either side of MLP + DROP on the head geometry (256x256x96, MSC+SM,
+    block 60000) with n histories: PCTN version-1 files written, read (read_data_chunk_v0, DS.cu:4361) and
+    uploaded; the FBP median filter (radius 3, DS.cu:6150-6166) inside the preprocessing images; the
+    preprocessed-data export and import (V.cu:6613-6658, :6775-6861); one DROP iteration with and without
+    the NTVS iteration before it (DS.cu:9889-9915); the text, .dat and PNG image writers (DS.cu:10743,
+    :9939).  Every call synchronises its stream before returning; file stages include the file system."""
+    import shutil
+    import tempfile
+    from pct_reconstruction_amd import Reconstruction, configs, io, synth
+
+    def timed(fn, *a):
+        t0 = time.perf_counter()
+        out = fn(*a)
+        return out, time.perf_counter() - t0
+
+    def dir_bytes(d):
+        return sum(f.stat().st_size for f in Path(d).rglob("*") if f.is_file())
+
+    P, ph, bm, n = configs.head_50m(n)
+    data = synth.generate(ph, bm, seed, n)
+    work = Path(tempfile.mkdtemp(prefix="pct_stages_"))
+    res = {"workload": f"head geometry 256x256x96, {n} histories", "histories": n}
+    try:
+        proj = work / "proj"
+        proj.mkdir()
+        parts = np.array_split(np.arange(n), files)
+        t0 = time.perf_counter()
+        for k, idx in enumerate(parts):
+            io.write_projection(proj / f"projection_{k:03d}.bin", data["t"][:, idx], data["v"][:, idx],
+                                data["u"][:, idx], data["wepl"][idx], float(k), angles=data["angle_deg"][idx])
+        t_write = time.perf_counter() - t0
+        nbytes = dir_bytes(proj)
+        chunks, t_read = timed(lambda: [io.read_projection(f) for f in sorted(proj.iterdir())])
+        with Reconstruction(P) as r:
+            t0 = time.perf_counter()
+            for c in chunks:
+                r.load_histories(c["t"], c["v"], c["u"], c["wepl"], c["angle_deg"])
+            r.synchronize()
+            t_load = time.perf_counter() - t0
+        del chunks
+        res["projection_files"] = {"files": files, "bytes": nbytes, "write_s": t_write, "read_s": t_read,
+                                   "upload_s": t_load, "read_GBs": nbytes / t_read / 1e9,
+                                   "read_histories_per_s": n / t_read, "upload_histories_per_s": n / t_load}
+        shutil.rmtree(proj)
+        Pm = P.copy()
+        Pm.median_filter_fbp, Pm.fbp_med_filter_radius = 1, 3
+        pre = {}
+        for name, Q in (("plain", P), ("median_r3", Pm)):
+            with Reconstruction(Q) as r:
+                r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+                r.import_and_process_data()
+                r.hull_detection_finish()
+                r.statistical_calculations_and_cuts()
+                pre[name] = min(timed(r.generate_preprocessing_images)[1] for _ in range(3))
+        res["fbp_median_filter"] = {"preprocessing_images_s": pre["plain"],
+                                    "preprocessing_images_median_r3_s": pre["median_r3"],
+                                    "median_filter_ms": (pre["median_r3"] - pre["plain"]) * 1e3}
+        exp = work / "preprocessed"
+        with Reconstruction(P) as r:
+            r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+            r.preprocessing()
+            r.reconstruction_cuts()
+            r.build_mlp_lists()
+            m, entries = r.count("reconstruction"), r.count("list_entries")
+            _, t_exp = timed(r.export_preprocessed, exp)
+            x_img = r.image("x")
+        ebytes = dir_bytes(exp)
+        with Reconstruction(P) as q:
+            _, t_imp = timed(q.import_preprocessed, exp)
+        res["preprocessed"] = {"reconstruction_histories": m, "list_entries": entries, "bytes": ebytes,
+                               "export_s": t_exp, "import_s": t_imp, "export_GBs": ebytes / t_exp / 1e9,
+                               "import_GBs": ebytes / t_imp / 1e9}
+        Pt = P.copy()
+        Pt.tvs_on = 1
+        drop, tvs_log = {}, []
+        for name, Q in (("drop", P), ("drop_tvs", Pt)):
+            with Reconstruction(Q) as r:
+                r.import_preprocessed(exp)
+                drop[name] = min(timed(r.DROP, 1)[1] for _ in range(3))
+                if name == "drop_tvs":
+                    tvs_log = r.tvs_log()
+        res["tvs"] = {"drop_iteration_ms": drop["drop"] * 1e3, "drop_iteration_with_ntvs_ms": drop["drop_tvs"] * 1e3,
+                      "ntvs_iteration_ms": (drop["drop_tvs"] - drop["drop"]) * 1e3,
+                      "ntvs_L_after": [L for _, _, L in tvs_log]}
+        shutil.rmtree(exp)
+        wr = {}
+        for name, fn, suffix in (("text", io.write_image_text, "txt"), ("dat", io.write_image_dat, "dat"),
+                                 ("png", io.write_image_png, "png")):
+            path = work / f"x.{suffix}"
+            wr[name] = {"s": timed(fn, path, x_img)[1], "bytes": path.stat().st_size}
+        res["writers"] = {"voxels": int(x_img.size), **wr}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return res
+
+
 def time_to_residual(r, P, weights, ref_iterations: int) -> dict:
     """FBP start: `ref_iterations` DROP iterations give the residual target ||Ax - b||.  Learned start:
     one DROP iteration, the net on that iterate (kept only if it lowers the residual), then DROP until
@@ -249,6 +346,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the time-to-residual measurement")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle comparison")
+    ap.add_argument("--stages-around", type=int, default=0, metavar="N",
+                    help="also time the stages around MLP + DROP (file I/O, median filter, preprocessed export and "
+                         "import, TVS, image writers) on N head-geometry histories")
     ap.add_argument("--fp64-pipeline", type=int, default=0, metavar="N",
                     help="also compare every image with the FP64 oracle of the whole pipeline on N fresh histories "
                          "of the workload (minutes of CPU time for N = 2M)")
@@ -431,6 +531,10 @@ def main() -> int:
     if learned is not None:
         learned.close()
     r.close()
+    around = None
+    if args.stages_around > 0:
+        log(f"stages around the path on {args.stages_around} histories")
+        around = stages_around(args.stages_around, args.seed)
     fp64_pipe = None
     if args.fp64_pipeline > 0:
         log(f"FP64 oracle of the whole pipeline on {args.fp64_pipeline} histories")
@@ -463,6 +567,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "fp64_oracle": fp64,
         "fp64_pipeline": fp64_pipe,
+        "stages_around": around,
         "hbm_step": hbm_step,
         "time_to_residual": warm,
         "learned_start": judges[-1] if judges else None,
