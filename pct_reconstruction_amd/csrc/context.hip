@@ -117,6 +117,19 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
         int f = std::atoi(e);
         if (f >= 0 && f <= 63) ctx->drop_flags = f;
     }
+    // hull carving: LDS hash slots of the fused walk, or 0 for one walk per method (the counts are the same)
+    if (const char* e = std::getenv("PCT_CARVE_TABLE")) {
+        int t = std::atoi(e);
+        if (t == 0 || t == 4096 || t == 8192 || t == 16384) ctx->carve_table = t;
+    }
+    if (const char* e = std::getenv("PCT_CARVE_BLOCK")) {
+        int t = std::atoi(e);
+        if (t == 256 || t == 512) ctx->carve_block = t;
+    }
+    if (const char* e = std::getenv("PCT_CARVE_SPILL")) {
+        int t = std::atoi(e);
+        if (t >= 1 && t <= 512) ctx->carve_spill = t;
+    }
     if (const char* e = std::getenv("PCT_DROP_GROUP_ENTRIES")) {
         long long g = std::atoll(e);
         if (g >= 1 && g <= (1ll << 30)) ctx->drop_group_entries = g;

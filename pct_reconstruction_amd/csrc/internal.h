@@ -170,6 +170,10 @@ struct pct_ctx {
     uint32_t* u32_d = nullptr;
     void* cub_tmp = nullptr;
     size_t cub_tmp_bytes = 0;
+    int64_t bin_perm_n = -1;  // u32_d holds the bin order of the first bin_perm_n histories (pct_bin_order)
+    int carve_spill = 512;    // LDS count at which a carve visit goes to global memory, 1..512 (PCT_CARVE_SPILL)
+    int carve_table = 8192;   // LDS hash slots of the fused carve (4096, 8192, 16384); 0: one walk per method (PCT_CARVE_TABLE)
+    int carve_block = 512;    // lanes per workgroup of the fused carve (256, 512; PCT_CARVE_BLOCK)
 
     // bins
     int32_t* counts_pre = nullptr;
@@ -315,6 +319,7 @@ int pct_stage_total_variation(pct_ctx* ctx, double* tv);  // TV of ctx->x, fixed
 int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bool with_first_voxel);
 // per-history arrays (and their compaction scratch) for n histories
 int pct_ensure_hist_capacity(pct_ctx* ctx, int64_t n);
+int pct_bin_order(pct_ctx* ctx, int64_t n);
 // per-kernel timing of DROP launches (HIP events around every launch on the launch stream)
 struct DropTiming {
     double scatter_ms = 0, apply_ms = 0;

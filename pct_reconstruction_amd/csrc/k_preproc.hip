@@ -299,6 +299,7 @@ __global__ void k_bin_stats(int NB, const int32_t* off, const uint32_t* perm, co
 int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bool with_first_voxel)
 {
     hipStream_t s = ctx->stream;
+    ctx->bin_perm_n = -1;  // the histories move
     if (n == 0) {
         *n_out = 0;
         return PCT_OK;
@@ -340,6 +341,40 @@ int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bo
     return PCT_OK;
 }
 
+// Stable sort of the indices of the first n histories by bin number into ctx->u32_d (an LSD radix sort keeps
+// history order inside a bin).  The order stays valid until the next compaction or binning.
+int pct_bin_order(pct_ctx* ctx, int64_t n)
+{
+    if (ctx->bin_perm_n == n) return PCT_OK;
+    hipStream_t s = ctx->stream;
+    if (n > 0x7fffffff) {
+        ctx->err = "bin order: more than 2^31 histories";
+        return PCT_E_LIMIT;
+    }
+    int end_bit = 1;
+    while ((1ll << end_bit) < (int64_t)ctx->P.num_bins) end_bit++;
+    uint32_t* keys_in = ctx->u32_a;
+    uint32_t* keys_out = ctx->u32_b;
+    uint32_t* vals_in = ctx->u32_c;
+    uint32_t* perm = ctx->u32_d;
+    if (n > 0) {
+        PCT_CHECK_HIP(ctx, hipMemcpyAsync(keys_in, ctx->bin_num, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(kBlock), 0, s, n, vals_in);
+        size_t need = 0;
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys_in, keys_out, vals_in, perm, (int)n, 0,
+                                                              end_bit, s));
+        if (need > ctx->cub_tmp_bytes) {
+            PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
+            ctx->cub_tmp_bytes = need;
+        }
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, need, keys_in, keys_out, vals_in, perm,
+                                                              (int)n, 0, end_bit, s));
+        PCT_CHECK_HIP(ctx, hipGetLastError());
+    }
+    ctx->bin_perm_n = n;
+    return PCT_OK;
+}
+
 int pct_ensure_hist_capacity(pct_ctx* ctx, int64_t n)
 {
     if (n <= ctx->cap_hist) return PCT_OK;
@@ -374,6 +409,7 @@ int pct_stage_intersect_bin_carve(pct_ctx* ctx)
     hipStream_t s = ctx->stream;
     const int64_t n = ctx->n_loaded;
     PCT_TRY(pct_ensure_hist_capacity(ctx, n));
+    ctx->bin_perm_n = -1;
     unsigned g = grid_for(n);
     RawPtrs r{ctx->raw[0], ctx->raw[1], ctx->raw[2], ctx->raw[3], ctx->raw[4], ctx->raw[5], ctx->raw[6],
               ctx->raw[7], ctx->raw[8], ctx->raw[9], ctx->raw[10], ctx->raw[11], ctx->raw[13]};
@@ -435,29 +471,12 @@ int pct_stage_stats_and_cuts(pct_ctx* ctx)
     }
     PCT_TRY(pct_alloc(ctx, &ctx->sinogram, NB));
     unsigned g = grid_for(n);
-    // stable sort of history indices by bin (LSD radix sort preserves history order inside a bin)
-    int end_bit = 1;
-    while ((1ll << end_bit) < (int64_t)NB) end_bit++;
-    uint32_t* keys_in = ctx->u32_a;
-    uint32_t* keys_out = ctx->u32_b;
-    uint32_t* vals_in = ctx->u32_c;
-    uint32_t* perm = ctx->u32_d;
+    // history indices in bin order (kept from the hull carving when no compaction came between)
+    PCT_TRY(pct_bin_order(ctx, n));
+    const uint32_t* perm = ctx->u32_d;
     int32_t* hist = ctx->counts_post;  // bin occupancy, reused below
     PCT_CHECK_HIP(ctx, hipMemsetAsync(hist, 0, NB * sizeof(int32_t), s));
-    if (n > 0) {
-        PCT_CHECK_HIP(ctx, hipMemcpyAsync(keys_in, ctx->bin_num, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_iota, dim3(g), dim3(kBlock), 0, s, n, vals_in);
-        size_t need = 0;
-        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys_in, keys_out, vals_in, perm, (int)n, 0,
-                                                              end_bit, s));
-        if (need > ctx->cub_tmp_bytes) {
-            PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
-            ctx->cub_tmp_bytes = need;
-        }
-        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, need, keys_in, keys_out, vals_in, perm,
-                                                              (int)n, 0, end_bit, s));
-        hipLaunchKernelGGL(k_bin_hist, dim3(g), dim3(kBlock), 0, s, n, ctx->bin_num, hist);
-    }
+    if (n > 0) hipLaunchKernelGGL(k_bin_hist, dim3(g), dim3(kBlock), 0, s, n, ctx->bin_num, hist);
     size_t need = 0;
     PCT_CHECK_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, hist, ctx->bin_off, NB + 1, s));
     if (need > ctx->cub_tmp_bytes) {

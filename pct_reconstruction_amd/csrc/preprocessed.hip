@@ -327,6 +327,7 @@ extern "C" int pct_import_preprocessed(pct_ctx* ctx, const char* dir)
         PCT_CHECK_HIP(ctx, hipMemcpy(ctx->chord, chord.data(), n * sizeof(double), hipMemcpyHostToDevice));
         PCT_CHECK_HIP(ctx, hipMemcpy(ctx->wepl, wepl.data(), n * sizeof(float), hipMemcpyHostToDevice));
         PCT_CHECK_HIP(ctx, hipMemcpy(ctx->bin_num, bin.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        ctx->bin_perm_n = -1;
         std::vector<float> g(gant.begin(), gant.end());
         PCT_CHECK_HIP(ctx, hipMemcpy(ctx->gantry, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
         if (!have_hist) {  // no entry voxels or coordinates: zeros, so that an export writes defined values

@@ -88,3 +88,31 @@ def test_reference_layout_files(tmp_path):
     lines = (tmp_path / "hull.txt").read_text().splitlines()
     assert len(lines) == P.rows * P.slices
     assert np.array_equal(np.array([ln.split() for ln in lines], float).reshape(-1), hull.reshape(-1))
+
+
+@pytest.mark.parametrize("table,block,spill", [("0", "256", "512"), ("4096", "512", "1"), ("8192", "256", "512"),
+                                               ("8192", "512", "512"), ("16384", "256", "3")])
+def test_carve_variants_count_identically(oracle, monkeypatch, table, block, spill):
+    """The hull counts of the fused, bin-ordered carve (LDS hash table of 4096/8192/16384 slots for 256 or 512
+    lanes, visits that find no slot or reach the spill count going to global atomics) and of one walk per
+    method (table 0) are the oracle's, bit for bit (hull_detection, DS.cu:6607; SC_GPU :6681, MSC_GPU :6799,
+    SM_GPU :6997)."""
+    from pct_reconstruction_amd import Reconstruction
+    monkeypatch.setenv("PCT_CARVE_TABLE", table)
+    monkeypatch.setenv("PCT_CARVE_BLOCK", block)
+    monkeypatch.setenv("PCT_CARVE_SPILL", spill)
+    P, data = small_head(n=60000, seed=37, sc_on=1, msc_on=1, sm_on=1)
+    ref = oracle.run_pipeline(P, data, stop_after="cuts")
+    with Reconstruction(P) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.import_and_process_data()
+        print(f"[carve table={table} block={block} spill={spill}] {r.count('recon_volume')} histories", flush=True)
+        assert np.array_equal(r.image("sc_hull").reshape(-1), ref["sc_hull"])
+        assert np.array_equal(r.image("msc_counts").reshape(-1), ref["msc_counts"])
+        assert np.array_equal(r.image("sm_counts").reshape(-1), ref["sm_counts"])
+        assert r.image("sm_counts").max() > 40  # many visits per voxel: the table aggregates
+        r.hull_detection_finish()
+        r.statistical_calculations_and_cuts()  # reuses the bin order of the carve
+        assert np.array_equal(r.bins("counts_post"), ref["counts_post"])
+        assert np.array_equal(r.bins("sinogram"), ref["sinogram_sum"])
+        assert r.count("post_cuts") == ref["h_cut"].shape[1]
