@@ -226,6 +226,56 @@ def time_to_residual(r, P, weights, ref_iterations: int) -> dict:
             "speedup": t_fbp / t_net, "weights": weights.name}
 
 
+def reference_scheme_drop(r, P, iterations: int) -> dict:
+    """The reference's DROP scheme on the same lists, through pct_drop_iteration_device (DROP_full_tx_iteration,
+    DS.cu:9537-9589): per block, one wavefront per history scatters x_update and S with two atomics per list
+    entry (calculate_x_update_GPU :9321-9419), then the touched voxels are applied (image_update_GPU
+    :9522-9536).  Timed with HIP events on the default stream; the image after the first iteration is compared
+    with the transposed DROP's from the same x0."""
+    import ctypes as C
+    import torch
+    from pct_reconstruction_amd._lib import lib
+    off, vox = r.lists()
+    chord, wepl = r.histories("chord"), r.histories("wepl")
+    x0 = r.image("x0").reshape(-1)
+    n, nv, entries = off.shape[0] - 1, x0.size, int(off[-1])
+    r.reset_iterate()
+    r.DROP(1)
+    x_transposed = r.image("x").reshape(-1)
+    dev = torch.device("cuda:0")
+    d_off, d_vox = torch.from_numpy(off).to(dev), torch.from_numpy(vox.view(np.int32)).to(dev)
+    del vox
+    d_chord, d_w = torch.from_numpy(chord).to(dev), torch.from_numpy(wepl).to(dev)
+    d_x = torch.from_numpy(x0.copy()).to(dev)
+    d_acc = torch.zeros(nv, dtype=torch.float64, device=dev)
+    d_cnt = torch.zeros(nv, dtype=torch.int32, device=dev)
+    f = lib().pct_drop_iteration_device
+    args = (n, C.c_void_p(d_off.data_ptr()), C.c_void_p(d_vox.data_ptr()), C.c_void_p(d_chord.data_ptr()),
+            C.c_void_p(d_w.data_ptr()), C.c_void_p(d_x.data_ptr()), C.c_void_p(d_acc.data_ptr()),
+            C.c_void_p(d_cnt.data_ptr()), nv, P.drop_block_size, P.histories_per_block, P.quirk_a3_drop_floor,
+            P.min_mlp_length, P.lambda_, None)
+    torch.cuda.synchronize()
+    times, rel = [], None
+    for k in range(iterations):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = f(*args)
+        e1.record()
+        torch.cuda.synchronize()
+        if rc != 0:
+            raise RuntimeError(f"pct_drop_iteration_device returned {rc}")
+        times.append(e0.elapsed_time(e1))
+        if k == 0:
+            x1 = d_x.cpu().numpy()
+            rel = float(np.linalg.norm(x1 - x_transposed) / np.linalg.norm(x_transposed))
+    ms = float(np.median(times))
+    return {"scheme": "two atomics per list entry (x_update as FP64 sums, S as u32), touched-voxel apply",
+            "iterations": iterations, "ms_per_iteration": ms, "ms_each": times, "histories": n,
+            "list_entries": entries, "entries_per_s": entries / (ms * 1e-3),
+            "scatter_atomics_per_s": 2 * entries / (ms * 1e-3),
+            "rel_l2_vs_transposed_iteration_1": rel}
+
+
 def fp64_oracle_error(r, P, iterations: int) -> dict:
     """The image after `iterations` DROP iterations from x0 against the oracle's FP64 DROP (x, sums and scalars in
     double) over the library's lists, which the parity tests hold bit-identical to the oracle's own."""
@@ -352,6 +402,8 @@ def main() -> int:
     ap.add_argument("--fp64-pipeline", type=int, default=0, metavar="N",
                     help="also compare every image with the FP64 oracle of the whole pipeline on N fresh histories "
                          "of the workload (minutes of CPU time for N = 2M)")
+    ap.add_argument("--reference-scheme", type=int, default=0, metavar="K",
+                    help="also time K iterations of the reference's atomic DROP scheme on the same lists")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -508,6 +560,13 @@ def main() -> int:
         fp64 = fp64_oracle_error(r, r.params, 2)
         log(f"FP64 oracle: rel L2 {fp64['rel_l2']:.3e} after {fp64['iterations']} iterations")
 
+    ref_scheme = None
+    if args.reference_scheme > 0 and not segs:
+        log(f"reference DROP scheme, {args.reference_scheme} iterations")
+        ref_scheme = reference_scheme_drop(r, r.params, args.reference_scheme)
+        log(f"reference scheme: {ref_scheme['ms_per_iteration']:.1f} ms per iteration, "
+            f"{ref_scheme['scatter_atomics_per_s'] / 1e9:.1f} G atomics/s")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -570,6 +629,7 @@ def main() -> int:
         "stages_around": around,
         "hbm_step": hbm_step,
         "time_to_residual": warm,
+        "reference_scheme_drop": ref_scheme,
         "learned_start": judges[-1] if judges else None,
         "kernels": kernels,
         "stages_ms": stages,
