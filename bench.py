@@ -10,8 +10,10 @@ metric: MLP+DROP proton histories/s = reconstruction histories x DROP iterations
 The JSON line also carries the roofline of the dominant kernel (HIP-event timing of every launch of
 that kernel during one profiled iteration) and the oracle's CPU rate on a bounded sample.
 
-time_to_residual: on head-50M, the wall time of 20 FBP-start DROP iterations against the learned start
-(1 iteration, the shipped net on that iterate, DROP until the same residual), with the iteration counts.
+time_to_residual: on head-50M, the wall time from the FBP x0 to the residual of 20 DROP iterations against
+the learned start (the shipped net on the FBP image and the hull, then DROP until the same residual), with
+the residual checked after every iteration inside the timed region, so that the iteration counts are found
+there; the protocol with the net on the iterate after one DROP iteration is reported beside it.
 
 --workload large-200M-segment measures one GPU's share of the large-200M run instead: a 25M-history
 segment on the 512x512x128 grid (not the default line).
@@ -185,45 +187,157 @@ def stages_around(n: int, seed: int, files: int = 10) -> dict:
     return res
 
 
-def time_to_residual(r, P, weights, ref_iterations: int) -> dict:
-    """FBP start: `ref_iterations` DROP iterations give the residual target ||Ax - b||.  Learned start:
-    one DROP iteration, the net on that iterate (kept only if it lowers the residual), then DROP until
-    the target is reached.  Both timed from x0 with the lists built; the learned time includes the
-    net's inference and residual judgement."""
+def time_to_residual(r, P, weights, ref_iterations: int, fp64: bool) -> dict:
+    """Time to the FBP-start residual from x0, for both starts, with the iteration counts found inside the timed
+    region.  Target: ||Ax - b|| after `ref_iterations` DROP iterations from the FBP x0 (an untimed pass that
+    defines it).  FBP start: DROP iterations, the residual after each, until it is at or below the target.
+    Learned start: one DROP iteration, the net on that iterate (kept only if its residual is lower), then DROP
+    iterations, the residual after each, until the target (at least one after the net, so that DROP defines
+    the final image).  Both timings include every residual evaluation; replays with the iteration counts
+    fixed (no residuals) are reported beside them.  With `fp64`, the learned start's final image is compared
+    with the oracle's FP64 run of the same protocol (FP64 DROP, FP64 network) over the same lists."""
+    from pct_reconstruction_amd.learned import WarmStartNet
+
+    def clock():
+        r.synchronize()
+        return time.perf_counter()
+
+    net = WarmStartNet(P.columns, P.rows, P.slices, seed=0)
+    cap = 2 * ref_iterations
+    try:
+        net.load(str(weights))
+        r.reset_iterate()
+        r.DROP(ref_iterations)
+        target = r.residual()
+        hull = r.image("hull").reshape(-1) > 0
+        x_fbp = r.image("x").reshape(-1)
+        # FBP start, residual checked after every iteration
+        r.reset_iterate()
+        t0 = clock()
+        k_fbp, res_fbp = 0, float("inf")
+        while k_fbp < cap and res_fbp > target:
+            r.DROP(1)
+            k_fbp += 1
+            res_fbp = r.residual()
+        t_fbp = clock() - t0
+        # learned start, residual checked after every iteration
+        r.reset_iterate()
+        t0 = clock()
+        r.DROP(1)
+        judge = net.apply_iterate(r)
+        k_net = 1
+        while True:
+            r.DROP(1)
+            k_net += 1
+            res_net = r.residual()
+            if res_net <= target or k_net >= cap:
+                break
+        t_net = clock() - t0
+        x_net = r.image("x").reshape(-1)
+        # replays with the counts fixed
+        r.reset_iterate()
+        t0 = clock()
+        r.DROP(k_fbp)
+        t_fbp_fixed = clock() - t0
+        r.reset_iterate()
+        t0 = clock()
+        r.DROP(1)
+        net.apply_iterate(r)
+        r.DROP(k_net - 1)
+        t_net_fixed = clock() - t0
+        params = net.get_params()
+    finally:
+        net.close()
+    d = x_net - x_fbp
+    out = {"reference": f"FBP x0, {ref_iterations} DROP iterations", "residual_target": target,
+           "residual_fbp": res_fbp, "residual_reached": res_net, "iterations_fbp": k_fbp,
+           "iterations_learned": k_net, "net_used": judge["used_net"],
+           "fbp_ms": 1e3 * t_fbp, "learned_ms": 1e3 * t_net, "speedup": t_fbp / t_net,
+           "counts_found_in_timed_region": True,
+           "fixed_counts": {"fbp_ms": 1e3 * t_fbp_fixed, "learned_ms": 1e3 * t_net_fixed,
+                            "speedup": t_fbp_fixed / t_net_fixed},
+           "learned_vs_fbp_start_image_rel_l2_hull": float(np.linalg.norm(d[hull]) / np.linalg.norm(x_fbp[hull])),
+           "weights": weights.name}
+    if fp64:
+        out["fp64_oracle"] = learned_fp64_error(r, P, params, judge["used_net"], k_net, x_net)
+    return out
+
+
+def fbp_input_start(r, P, weights, warm: dict, fp64: bool) -> dict:
+    """The network on the FBP image and the hull (pct_net_apply: the net's x0 replaces the FBP x0 when its
+    residual is lower), then DROP iterations, the residual after each, until the FBP start's residual target
+    (at least one).  Timed like the FBP start in `warm`, against which the speedup is taken.  The context's
+    x0 becomes the net's, so this runs last.  With `fp64`, the final image is compared with the oracle's FP64
+    run of the same protocol over the same lists."""
     from pct_reconstruction_amd.learned import WarmStartNet
     net = WarmStartNet(P.columns, P.rows, P.slices, seed=0)
+    target, cap = warm["residual_target"], 2 * warm["iterations_fbp"]
+    fbp = r.image("fbp")
     try:
         net.load(str(weights))
         r.reset_iterate()
         r.synchronize()
         t0 = time.perf_counter()
-        r.DROP(ref_iterations)
-        r.synchronize()
-        t_fbp = time.perf_counter() - t0
-        target = r.residual()
-        r.reset_iterate()
-        r.DROP(1)
-        judge = net.apply_iterate(r)
-        k = 1
-        res = r.residual()
-        while k < 2 or (res > target and k < 2 * ref_iterations):
+        judge = net.apply(r)
+        k = 0
+        while True:
             r.DROP(1)
             k += 1
             res = r.residual()
-        r.reset_iterate()
+            if res <= target or k >= cap:
+                break
         r.synchronize()
-        t0 = time.perf_counter()
-        r.DROP(1)
-        net.apply_iterate(r)
-        r.DROP(k - 1)
-        r.synchronize()
-        t_net = time.perf_counter() - t0
+        t = time.perf_counter() - t0
+        params = net.get_params()
     finally:
         net.close()
-    return {"reference": f"FBP x0, {ref_iterations} DROP iterations", "residual_target": target,
-            "residual_reached": res, "iterations_fbp": ref_iterations, "iterations_learned": k,
-            "net_used": judge["used_net"], "fbp_ms": 1e3 * t_fbp, "learned_ms": 1e3 * t_net,
-            "speedup": t_fbp / t_net, "weights": weights.name}
+    x_net = r.image("x").reshape(-1)
+    out = {"protocol": "net(FBP, hull) as x0 (pct_net_apply), then DROP", "net_used": judge["used_net"],
+           "residual_fbp_x0": judge["residual_fbp"], "residual_net_x0": judge["residual_net"],
+           "residual_reached": res, "iterations_learned": k, "learned_ms": 1e3 * t,
+           "speedup": warm["fbp_ms"] / (1e3 * t), "weights": weights.name}
+    if fp64:
+        out["fp64_oracle"] = learned_fp64_error(r, P, params, judge["used_net"], k, x_net, fbp=fbp)
+    return out
+
+
+def learned_fp64_error(r, P, params, used_net: bool, k: int, x_gpu, fbp=None) -> dict:
+    """The learned start's image against the oracle's FP64 run of the same protocol over the library's lists.
+    Without `fbp` (network on the iterate after one DROP iteration): FP64 DROP iteration from x0, the FP64
+    network on that iterate (rounded to float32, the image type the network reads), k - 1 FP64 DROP
+    iterations.  With `fbp` (network on the FBP image): the FP64 network on the FBP image, k FP64 DROP
+    iterations.  The GPU runs DROP with FP64 sums and the network in BF16 MFMA with FP32 accumulation."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pct_oracle as O
+    t0 = time.perf_counter()
+    off, vox = r.lists()
+    chord, w = r.histories("chord"), r.histories("wepl")
+    hull = r.image("hull").astype(np.uint8)
+    shape = (P.slices, P.rows, P.columns)
+
+    def net64(img):
+        _, _, y = O.net_loss_grad(shape, params, np.asarray(img, np.float32).reshape(shape), hull,
+                                  np.zeros(shape, np.float32), bf16=False, with_grad=False)
+        return y.reshape(-1)
+
+    if fbp is None:
+        x64 = O.drop_iteration_fp64(P, off, vox, chord, w, r.image("x0").reshape(-1).astype(np.float64))
+        if used_net:
+            x64 = net64(x64)
+        more = k - 1
+        protocol = (f"FP64 DROP x1, FP64 network {'applied' if used_net else 'judged and not applied'}, "
+                    f"{more} more FP64 DROP iterations")
+    else:
+        x64 = net64(fbp) if used_net else r.image("x0").reshape(-1).astype(np.float64)
+        more = k
+        protocol = f"FP64 network on the FBP image{'' if used_net else ' (not applied)'}, {k} FP64 DROP iterations"
+    for _ in range(more):
+        x64 = O.drop_iteration_fp64(P, off, vox, chord, w, x64)
+    del off, vox
+    x = np.asarray(x_gpu, np.float64).reshape(-1)
+    rel = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+    return {"rel_l2": rel, "target": 1e-3, "meets_target": rel <= 1e-3, "max_abs": float(np.abs(x - x64).max()),
+            "oracle_s": time.perf_counter() - t0, "protocol": protocol}
 
 
 def reference_scheme_drop(r, P, iterations: int) -> dict:
@@ -550,9 +664,12 @@ def main() -> int:
     warm = None
     weights = ROOT / "pct_reconstruction_amd" / "weights" / "warmstart_head50m_x1.bin"
     if not args.no_warm_start and args.workload == "head-50M" and weights.exists():
-        warm = time_to_residual(r, P, weights, P.iterations)
-        log(f"time to residual: FBP start {warm['fbp_ms']:.1f} ms, learned start {warm['learned_ms']:.1f} ms "
-            f"({warm['iterations_learned']} iterations), speedup {warm['speedup']:.2f}")
+        warm = time_to_residual(r, P, weights, P.iterations, not args.no_fp64)
+        log(f"time to residual: FBP start {warm['fbp_ms']:.1f} ms ({warm['iterations_fbp']} iterations), learned "
+            f"start {warm['learned_ms']:.1f} ms ({warm['iterations_learned']} iterations), speedup "
+            f"{warm['speedup']:.2f}")
+        if "fp64_oracle" in warm:
+            log(f"learned start vs FP64 oracle: rel L2 {warm['fp64_oracle']['rel_l2']:.3e}")
 
     fp64 = None
     if args.workload == "head-50M" and not args.no_fp64:
@@ -587,6 +704,19 @@ def main() -> int:
                       "drop": n_recon * (iters - 1 if learned is not None else iters)}
     stage_rates = {k: (w / (stages[k] * 1e-3) if stages.get(k) else None) for k, w in per_stage_work.items()}
     run_hull, run_x0 = r.image("hull").astype(np.uint8), r.image("x0")
+    fbp_weights = ROOT / "pct_reconstruction_amd" / "weights" / "warmstart_head50m_fbp.bin"
+    if warm is not None and fbp_weights.exists():
+        # the network on FBP + hull is the primary learned start; the iterate protocol is reported beside it
+        fi = fbp_input_start(r, P, fbp_weights, warm, not args.no_fp64)
+        x1 = {k: warm.pop(k) for k in ("residual_reached", "iterations_learned", "net_used", "learned_ms",
+                                       "speedup", "fixed_counts", "learned_vs_fbp_start_image_rel_l2_hull",
+                                       "weights", "fp64_oracle") if k in warm}
+        x1["protocol"] = "one DROP iteration, net(iterate, hull) (pct_net_apply_iterate), then DROP"
+        warm.update(fi)
+        warm["iterate_protocol"] = x1
+        log(f"FBP-input start: {fi['learned_ms']:.1f} ms ({fi['iterations_learned']} iterations), "
+            f"speedup {fi['speedup']:.2f}" + (f", FP64 oracle rel L2 {fi['fp64_oracle']['rel_l2']:.3e}"
+                                               if "fp64_oracle" in fi else ""))
     if learned is not None:
         learned.close()
     r.close()

@@ -68,6 +68,8 @@ static void conv_fwd(const double* a, int ca, int ac, const double* b, int cb, d
                      const double* bias, int relu, int cout, int bf, double* out)
 {
     const int cin = ca + cb;
+    /* every output is one sequential sum: the z-slabs are independent */
+#pragma omp parallel for schedule(dynamic, 1)
     for (int z = 0; z < d.D; z++)
         for (int y = 0; y < d.H; y++)
             for (int x = 0; x < d.W; x++) {

@@ -173,6 +173,7 @@ struct pct_ctx {
     int64_t bin_perm_n = -1;  // u32_d holds the bin order of the first bin_perm_n histories (pct_bin_order)
     int carve_spill = 512;    // LDS count at which a carve visit goes to global memory, 1..512 (PCT_CARVE_SPILL)
     int carve_table = 8192;   // LDS hash slots of the fused carve (4096, 8192, 16384); 0: one walk per method (PCT_CARVE_TABLE)
+    int endpoint_order = 1;   // MLP endpoints in sinogram-bin order (PCT_ENDPOINT_ORDER)
     int carve_block = 512;    // lanes per workgroup of the fused carve (256, 512; PCT_CARVE_BLOCK)
 
     // bins

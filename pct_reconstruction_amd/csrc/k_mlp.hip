@@ -129,12 +129,15 @@ struct H10 {
     float* h[10];
 };
 
-// collect_MLP_endpoints_GPU_nobool, DS.cu:7819-7854
-__global__ void k_endpoints(int64_t n, DevGeom g, const uint8_t* __restrict__ hull, H10 hp, uint32_t* first_voxel,
-                            uint8_t* keep)
+// collect_MLP_endpoints_GPU_nobool, DS.cu:7819-7854.  With `perm`, lane t takes history perm[t]: in
+// sinogram-bin order the lanes of a wavefront walk nearly the same chords, so their walks to the hull take
+// about as many steps.
+__global__ void k_endpoints(int64_t n, const uint32_t* __restrict__ perm, DevGeom g, const uint8_t* __restrict__ hull,
+                            H10 hp, uint32_t* first_voxel, uint8_t* keep)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = perm ? (int64_t)perm[t] : t;
     double xi = 0, yi = 0, zi = 0, xo = 0, yo = 0, zo = 0;
     int vx, vy, vz, vxi, vyi, vzi;
     int entered = find_endpoint(g, hull, hp.h[0][i], hp.h[1][i], hp.h[2][i], hp.h[6][i], hp.h[7][i], &xi, &yi, &zi,
@@ -543,8 +546,13 @@ int pct_stage_endpoints(pct_ctx* ctx)
     DevGeom g = pct_geom(P);
     H10 hp;
     for (int k = 0; k < 10; k++) hp.h[k] = ctx->h[k];
+    const uint32_t* perm = nullptr;
+    if (n > 0 && ctx->endpoint_order) {
+        PCT_TRY(pct_bin_order(ctx, n));
+        perm = ctx->u32_d;
+    }
     if (n > 0)
-        hipLaunchKernelGGL(k_endpoints, dim3(grid_for(n)), dim3(kBlock), 0, s, n, g,
+        hipLaunchKernelGGL(k_endpoints, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, g,
                            ctx->hull_endpoints ? ctx->hull_endpoints : ctx->hull, hp, ctx->first_voxel,
                            ctx->flag);
     PCT_CHECK_HIP(ctx, hipGetLastError());
