@@ -232,6 +232,43 @@ __global__ void k_mlp_capacity(int64_t n, MlpArgs a, H10 hp, const double* __res
 }
 
 // find_MLP_path_GPU_tabulated, DS.cu:9104-9262, writing the voxel list of history i.
+// A lane's list entries go out in whole 64-byte chunks: the lane collects them in its own LDS column
+// (entry k at buf[k * kBlock], so the lanes of a wavefront use distinct banks) and writes each chunk that
+// ends on a 16-entry boundary of the list array with four 16-byte stores, the partial chunks at either end
+// of its list entry by entry.  Written one 4-byte entry per step instead, the lines of 64 interleaved lists
+// leave the L2 partly written and the HBM write traffic grows about tenfold.
+struct ListWriter {
+    uint32_t* vox;   // the list array (16-entry chunks are 64-byte aligned: hipMalloc aligns to 256 bytes)
+    uint32_t* buf;   // this lane's LDS column
+    int64_t start;   // list index of buf[0]
+    int n;           // buffered entries, at most 16
+
+    __device__ __forceinline__ ListWriter(uint32_t* v, uint32_t* lds, int64_t base)
+        : vox(v), buf(lds + threadIdx.x), start(base), n(0)
+    {
+    }
+    __device__ __forceinline__ void put(uint32_t x)
+    {
+        buf[n * kBlock] = x;
+        ++n;
+        if (((start + n) & 15) == 0) flush();
+    }
+    __device__ __forceinline__ void flush()
+    {
+        if (n == 16) {  // ends on a 16-entry boundary with 16 entries: starts on one too
+            uint4* d = reinterpret_cast<uint4*>(vox + start);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                d[q] = make_uint4(buf[(4 * q) * kBlock], buf[(4 * q + 1) * kBlock], buf[(4 * q + 2) * kBlock],
+                                  buf[(4 * q + 3) * kBlock]);
+        } else {
+            for (int k = 0; k < n; k++) vox[start + k] = buf[k * kBlock];
+        }
+        start += n;
+        n = 0;
+    }
+};
+
 __global__ void __launch_bounds__(kBlock) k_mlp_lists(int64_t n, MlpArgs a, H10 hp, const uint32_t* __restrict__ first,
                                                       const double* __restrict__ sin_t,
                                                       const double* __restrict__ cos_t,
@@ -243,6 +280,7 @@ __global__ void __launch_bounds__(kBlock) k_mlp_lists(int64_t n, MlpArgs a, H10 
                                                       int32_t* overflow)
 {
     extern __shared__ double lds_terms[];  // n_steps x 6
+    __shared__ uint32_t list_buf[16 * kBlock];
     for (int k = threadIdx.x; k < a.n_steps; k += blockDim.x) {
 #pragma unroll
         for (int c = 0; c < 6; c++) lds_terms[k * 6 + c] = step_terms[k * 8 + c];
@@ -266,8 +304,9 @@ __global__ void __launch_bounds__(kBlock) k_mlp_lists(int64_t n, MlpArgs a, H10 
         const double u2_sq = (double)pct_sqf(u_2);
         const int64_t base = off[i], capn = off[i + 1] - off[i];
         uint32_t last = first[i];
-        int64_t cnt = 0;
-        vox[base + cnt++] = last;
+        int64_t cnt = 1;
+        ListWriter w(vox, list_buf, base);
+        w.put(last);
         chord[i] = chord_length(g.PI, g.VW, g.VT, STEP, (f.xy_entry + f.xy_exit) / 2.0, (xz_entry + xz_exit) / 2.0);
         int k = 0;
         while (depth > STEP) {
@@ -309,13 +348,15 @@ __global__ void __launch_bounds__(kBlock) k_mlp_lists(int64_t n, MlpArgs a, H10 
                     atomicOr(overflow, 2);
                     break;
                 }
-                vox[base + cnt++] = (uint32_t)voxel;
+                w.put((uint32_t)voxel);
+                cnt++;
                 last = (uint32_t)voxel;
             }
             depth -= STEP;
             s2 -= a.s_step;
             k++;
         }
+        w.flush();
         cnt_out[i] = (uint32_t)cnt;
     }
 }
@@ -400,6 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_mlp_lists_std(int64_t n, StdArgs a, 
                                                           uint32_t* __restrict__ cnt_out, uint32_t* __restrict__ vox,
                                                           double* __restrict__ chord, int32_t* overflow)
 {
+    __shared__ uint32_t list_buf[16 * kBlock];
     const DevGeom& g = a.g;
     const double STEP = a.step, E0 = a.e0, X0 = a.x0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -421,8 +463,9 @@ __global__ void __launch_bounds__(kBlock) k_mlp_lists_std(int64_t n, StdArgs a, 
         const double R1T_0 = 1.0, R1T_1 = 0.0, R1T_3 = 1.0;
         const int64_t base = off[i], capn = off[i + 1] - off[i];
         uint32_t last = first[i];
-        int64_t cnt = 0;
-        vox[base + cnt++] = last;
+        int64_t cnt = 1;
+        ListWriter w(vox, list_buf, base);
+        w.put(last);
         double u_1 = STEP;
         int k = 0;
         while (u_1 < u_2 - STEP) {
@@ -485,12 +528,14 @@ __global__ void __launch_bounds__(kBlock) k_mlp_lists_std(int64_t n, StdArgs a, 
                     atomicOr(overflow, 2);
                     break;
                 }
-                vox[base + cnt++] = (uint32_t)voxel;
+                w.put((uint32_t)voxel);
+                cnt++;
                 last = (uint32_t)voxel;
             }
             u_1 += STEP;
             k++;
         }
+        w.flush();
         cnt_out[i] = (uint32_t)cnt;
     }
 }
