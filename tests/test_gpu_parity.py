@@ -230,3 +230,36 @@ def test_tiny_and_empty_drop_blocks(oracle):
             r.DROP(P.iterations)
             assert _mismatch(r.image("x").reshape(-1), x) == 0, launch
     print(f"[tiny blocks] rel L2 vs oracle {rel:.2e}; layouts bit-identical", flush=True)
+
+
+def test_row_sums_that_round_follow_the_reference_order(oracle):
+    """calculate_x_update_GPU sums a_i.x in list order (DS.cu:9245).  On an image of 1 and 2^-53 values the
+    order matters: 1 + 2^-53 rounds to 1 while 2^-53 + 2^-53 does not.  The row kernel's regrouped sums are
+    used only where no partial sum can round; other rows are summed again in list order, so the DROP image
+    matches the oracle's ordered sums bit for bit."""
+    import math
+    from pct_reconstruction_amd import Reconstruction
+    P, data = small_cylinder(n=30000, seed=12, iterations=1)
+    ref = oracle.run_pipeline(P, data, stop_after="lists")
+    off, vox, chord, w = ref["offsets"], ref["voxels"], ref["chord"], ref["wepl_recon"]
+    x = np.where(np.arange(P.num_voxels) % 3 == 0, 1.0, 2.0 ** -53).astype(np.float32)
+    rows = np.arange(off.size - 1)[np.diff(off) >= P.min_mlp_length][:2000]
+    rounded = sum(float(np.cumsum(x[vox[off[i]:off[i + 1]]].astype(np.float64))[-1])
+                  != math.fsum(x[vox[off[i]:off[i + 1]]].astype(np.float64)) for i in rows)
+    print(f"[ordered row sums] {rounded} of {rows.size} rows round in list order", flush=True)
+    assert rounded > 100
+    x_ref = oracle.drop_iteration(P, off, vox, chord, w, x, accum=P.drop_accum)
+    with Reconstruction(P) as r:
+        r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
+        r.preprocessing()
+        r.reconstruction_cuts()
+        r.build_mlp_lists()
+        g_off, g_vox = r.lists()
+        assert np.array_equal(g_off, off) and np.array_equal(g_vox, vox)
+        r.set_image(x)
+        r.DROP(1)
+        assert _mismatch(r.image("x").reshape(-1), x_ref) == 0
+        r.set_image(x)
+        res = r.residual()
+    res_ref = oracle.residual(off, vox, chord, w, x, P.min_mlp_length)
+    assert abs(res - res_ref) <= 1e-12 * res_ref
