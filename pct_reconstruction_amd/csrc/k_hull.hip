@@ -374,8 +374,9 @@ int pct_carve_counts(pct_ctx* ctx)
         hipLaunchKernelGGL(k_init_counts, dim3(grid_for(NV)), dim3(kBlock), 0, s, g, ctx->sm_counts);
     }
     if (n > 0 && (P.sc_on || P.msc_on || P.sm_on)) {
-        if (ctx->carve_table && NV < (int64_t)kEmpty) {
-            // histories in sinogram-bin order; voxel numbers are 32-bit table keys below kEmpty
+        if (ctx->carve_table && NV < (int64_t)kEmpty && n <= 0x7fffffff) {
+            // histories in sinogram-bin order (a sort of up to 2^31 - 1); voxel numbers are 32-bit table keys
+            // below kEmpty
             PCT_TRY(pct_bin_order(ctx, n));
             Gates gt{P.sc_on, P.msc_on, P.sm_on, P.sc_lower_threshold, P.sc_upper_threshold, P.msc_lower_threshold,
                      P.msc_upper_threshold, P.sm_lower_threshold, (uint32_t)ctx->carve_spill};

@@ -592,7 +592,7 @@ int pct_stage_endpoints(pct_ctx* ctx)
     H10 hp;
     for (int k = 0; k < 10; k++) hp.h[k] = ctx->h[k];
     const uint32_t* perm = nullptr;
-    if (n > 0 && ctx->endpoint_order) {
+    if (n > 0 && n <= 0x7fffffff && ctx->endpoint_order) {  // the bin-order sort takes up to 2^31 - 1
         PCT_TRY(pct_bin_order(ctx, n));
         perm = ctx->u32_d;
     }
