@@ -131,6 +131,11 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
         int t = std::atoi(e);
         if (t >= 1 && t <= 512) ctx->carve_spill = t;
     }
+    if (const char* e = std::getenv("PCT_DROP_PIPELINE")) ctx->drop_pipeline = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PCT_MLP_BLOCKS_PER_CU")) {
+        int b = std::atoi(e);
+        if (b >= -1 && b <= 64) ctx->mlp_blocks_per_cu = b;
+    }
     if (const char* e = std::getenv("PCT_DROP_GROUP_ENTRIES")) {
         long long g = std::atoll(e);
         if (g >= 1 && g <= (1ll << 30)) ctx->drop_group_entries = g;
@@ -179,6 +184,7 @@ int pct_ctx_destroy(pct_ctx* ctx)
     pct_free(&ctx->tvs_G); pct_free(&ctx->tvs_part); pct_free(&ctx->tvs_res);
     if (ctx->tvs_res_h) (void)hipHostFree(ctx->tvs_res_h);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return PCT_OK;
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -139,6 +140,7 @@ struct pct_ctx {
     pct_params P;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // the MLP list kernels while the DROP plan is built (default priority)
     std::string err;
     std::string metrics;
     int stage = ST_EMPTY;
@@ -218,6 +220,8 @@ struct pct_ctx {
     int drop_row_lanes = 16;   // lanes per history in the row kernel (PCT_DROP_ROW_LANES: 4..64)
     int drop_col_voxels = 1;   // voxels per lane in the column kernel (PCT_DROP_COL_VOXELS: 1, 2, 4, 8)
     int drop_flags = PCT_DROP_SPATIAL_ORDER | PCT_DROP_BRICKS | PCT_DROP_COLS_SLICED;  // PCT_DROP_FLAGS
+    int mlp_blocks_per_cu = -1;  // MLP list kernel grid: workgroups per CU, 0 = one lane per history, -1 = auto (PCT_MLP_BLOCKS_PER_CU)
+    int drop_pipeline = 1;  // MLP lists of the next transposition groups built while a group is transposed (PCT_DROP_PIPELINE)
     int64_t drop_group_entries = 1ll << 30;  // list entries per transposition group (PCT_DROP_GROUP_ENTRIES)
     double* accum64 = nullptr;
     float* accum32 = nullptr;
@@ -301,7 +305,11 @@ int pct_stage_build_lists(pct_ctx* ctx);
 struct DropTiming;
 int pct_stage_drop(pct_ctx* ctx, int iterations, DropTiming* timing = nullptr);
 int pct_drop_entries(pct_ctx* ctx, int64_t* histories, int64_t* gathered, int64_t* scattered);
-int pct_drop_prepare(pct_ctx* ctx);
+// Launches the MLP list kernel for the histories [h0, h1) on a stream (k_mlp.hip).
+using MlpLaunch = std::function<int(int64_t h0, int64_t h1, hipStream_t s)>;
+// Builds the DROP plan from the lists.  With `mlp`, the lists are not built yet: the MLP kernel runs
+// group by group on ctx->stream2 while the earlier groups are transposed on ctx->stream.
+int pct_drop_prepare(pct_ctx* ctx, const MlpLaunch* mlp = nullptr);
 int pct_median_filter_fbp(pct_ctx* ctx);
 int pct_avg_filter_hull(pct_ctx* ctx);
 void pct_drop_plan_free(DropPlan* plan);

@@ -96,6 +96,16 @@ __global__ void __launch_bounds__(256) k_block_entries(int64_t BS, const int32_t
     if (threadIdx.x == 0) out[rb] = part[0];
 }
 
+// per raw block: the list capacity of its processed histories (an upper bound of its entries)
+__global__ void k_block_caps(int64_t n_raw, int64_t BS, const int32_t* __restrict__ proc_raw,
+                             const int64_t* __restrict__ off, int64_t* __restrict__ out)
+{
+    const int64_t rb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (rb >= n_raw) return;
+    const int64_t start = rb * BS;
+    out[rb] = off[start + proc_raw[rb]] - off[start];
+}
+
 // ---- transposition of a group of blocks (once per list build) ----
 // History i belongs to raw block i / BS (BS = DROP_BLOCK_SIZE) at in-block index i % BS and is
 // processed when that index is below the block's proc (quirk A3 may leave a tail unprocessed).
@@ -1020,11 +1030,12 @@ int transpose_group_buckets(pct_ctx* ctx, DropPlan& D, const DropPlan::Group& g,
 // (block, sinogram bin) gives the visiting order, and one radix sort of the (block, voxel) keys of
 // all its list entries, a run-length encoding and a scan give every block's touched voxels with
 // their histories in history order (LSD radix sorts are stable).
-int pct_drop_prepare(pct_ctx* ctx)
+int pct_drop_prepare(pct_ctx* ctx, const MlpLaunch* mlp)
 {
     const pct_params& P = ctx->P;
     hipStream_t s = ctx->stream;
     const int64_t n = ctx->n_recon;
+    const bool pipe = mlp != nullptr;  // the lists are built group by group below
     const int64_t BS = P.drop_block_size;
     DropPlan& D = ctx->plan;
     D.blocks.clear();
@@ -1066,12 +1077,15 @@ int pct_drop_prepare(pct_ctx* ctx)
     std::vector<int64_t> e_raw(n_raw > 0 ? n_raw : 1, 0);
     if (n_raw > 0) {
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(proc_buf.p, proc_raw.data(), n_raw * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_block_entries, dim3((unsigned)n_raw), dim3(256), 0, s, BS, proc_dev, ctx->list_cnt,
-                           P.min_mlp_length, e_dev);
+        if (pipe)  // list capacities bound the entries until the lists exist: groups and allocations use them
+            hipLaunchKernelGGL(k_block_caps, dim3(grid_for(n_raw)), dim3(kBlock), 0, s, n_raw, BS, proc_dev,
+                               ctx->list_off, e_dev);
+        else
+            hipLaunchKernelGGL(k_block_entries, dim3((unsigned)n_raw), dim3(256), 0, s, BS, proc_dev, ctx->list_cnt,
+                               P.min_mlp_length, e_dev);
         PCT_CHECK_HIP(ctx, hipMemcpyAsync(e_raw.data(), e_dev, n_raw * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
     }
-    pct_free(&e_buf.p);
     int64_t total = 0, max_proc = 0;
     for (int64_t rb = 0; rb < n_raw; rb++) {
         const int64_t start = rb * BS;
@@ -1159,11 +1173,11 @@ int pct_drop_prepare(pct_ctx* ctx)
             PCT_CHECK_HIP(ctx, hipMemsetAsync(D.x_b, 0, D.nv_b * sizeof(float), s));  // padding stays 0
         }
         const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
-        if (!D.vox_b_ready && ctx->list_cap > 0)  // lists built without the brick index
+        if (!pipe && !D.vox_b_ready && ctx->list_cap > 0)  // lists built without the brick index (per group with pipe)
             hipLaunchKernelGGL(k_bricks_list, dim3(pct_grid_capped(ctx->n_recon * 16, kBlock, 1u << 20)), dim3(kBlock), 0,
                                s, ctx->n_recon, ctx->list_off, ctx->list_cnt, ctx->list_vox, D.vox_b, bg,
                                (uint32_t)P.num_voxels);
-        D.vox_b_ready = true;
+        D.vox_b_ready = !pipe;
         lvox = D.vox_b;
     }
     // bucket transposition: tiles of 4 x 4 bricks per block
@@ -1221,9 +1235,79 @@ int pct_drop_prepare(pct_ctx* ctx)
     std::vector<std::vector<int32_t>> run_start_h(ng);
     std::vector<uint32_t> n_runs_h(ng, 0);
     int64_t run_off = 0;
+    // With `mlp`: the MLP kernel for each group's histories (and the histories of no group) is queued on
+    // stream2 with an event after each group; a group is transposed on `s` once its event has fired, while
+    // stream2 builds the lists of the next groups.  The events and stream2 are drained on every return.
+    struct MlpQueue {
+        pct_ctx* ctx;
+        std::vector<hipEvent_t> ev;
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        ~MlpQueue()
+        {
+            if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+            if (t0) (void)hipEventDestroy(t0);
+            if (t1) (void)hipEventDestroy(t1);
+        }
+    } q{ctx, {}};
+    if (pipe) {
+        if (!ctx->stream2) PCT_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+        hipStream_t s2 = ctx->stream2;
+        hipEvent_t ready = nullptr;  // the capacity pass, the scan and the buffers on `s` come first
+        PCT_CHECK_HIP(ctx, hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        q.ev.push_back(ready);
+        PCT_CHECK_HIP(ctx, hipEventRecord(ready, s));
+        PCT_CHECK_HIP(ctx, hipStreamWaitEvent(s2, ready, 0));
+        PCT_CHECK_HIP(ctx, hipEventCreate(&q.t0));
+        PCT_CHECK_HIP(ctx, hipEventCreate(&q.t1));
+        PCT_CHECK_HIP(ctx, hipEventRecord(q.t0, s2));
+        int64_t prev = 0;
+        for (int64_t gi = 0; gi <= ng; gi++) {
+            const int64_t h1 = gi < ng ? D.groups[gi].hist_end : n;
+            if (h1 > prev) PCT_TRY((*mlp)(prev, h1, s2));
+            prev = h1 > prev ? h1 : prev;
+            hipEvent_t e = nullptr;
+            PCT_CHECK_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            q.ev.push_back(e);
+            PCT_CHECK_HIP(ctx, hipEventRecord(e, s2));
+        }
+        PCT_CHECK_HIP(ctx, hipEventRecord(q.t1, s2));
+    }
+    int64_t entry_total = 0;  // entries of the groups transposed so far
     std::vector<int64_t> group_run_off(ng, 0);
     for (int64_t gi = 0; gi < ng; gi++) {
-        const DropPlan::Group& g = D.groups[gi];
+        DropPlan::Group& gw = D.groups[gi];
+        if (pipe) {
+            PCT_CHECK_HIP(ctx, hipStreamWaitEvent(s, q.ev[gi + 1], 0));
+            const int64_t gh0 = gw.hist_begin, gH = gw.hist_end - gw.hist_begin, grb0 = gh0 / BS;
+            if (bricks && gH > 0) {
+                const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
+                hipLaunchKernelGGL(k_bricks_list, dim3(pct_grid_capped(gH * 16, kBlock, 1u << 20)), dim3(kBlock), 0, s,
+                                   gH, ctx->list_off + gh0, ctx->list_cnt + gh0, ctx->list_vox, D.vox_b, bg,
+                                   (uint32_t)P.num_voxels);
+            }
+            // the group's entries, now that its lists exist
+            hipLaunchKernelGGL(k_block_entries, dim3((unsigned)gw.n_blocks), dim3(256), 0, s, BS, proc_dev + grb0,
+                               ctx->list_cnt + grb0 * BS, P.min_mlp_length, e_dev + grb0);
+            PCT_CHECK_HIP(ctx, hipMemcpyAsync(e_raw.data() + grb0, e_dev + grb0, gw.n_blocks * sizeof(int64_t),
+                                              hipMemcpyDeviceToHost, s));
+            PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
+            gw.entry_base = entry_total;
+            gw.entries = 0;
+            for (int64_t b = gw.first_block; b < nb && D.blocks[b].group == gi; b++) {
+                DropPlan::Block& bk = D.blocks[b];
+                const int64_t e = e_raw[bk.start / BS];
+                if (e > bk.entries) {  // the capacity bounds the entries
+                    ctx->err = "DROP transposition: more entries than list capacity in a block";
+                    return PCT_E_LIMIT;
+                }
+                bk.entries = e;
+                bk.entry_base = entry_total + gw.entries;
+                gw.entries += e;
+            }
+        }
+        entry_total += gw.entries;
+        const DropPlan::Group& g = gw;
         const int64_t h0 = g.hist_begin, H = g.hist_end - g.hist_begin, E = g.entries;
         const int64_t rb0 = h0 / BS;
         const int sort_bits = bits_for(g.n_blocks) + bin_bits;
@@ -1287,8 +1371,30 @@ int pct_drop_prepare(pct_ctx* ctx)
         }
         run_off += (int64_t)n_runs_h[gi];
     }
+    if (pipe) {
+        PCT_CHECK_HIP(ctx, hipStreamWaitEvent(s, q.ev[ng + 1], 0));  // the histories of no group
+        if (bricks) {  // their lists too hold brick indices (the residual reads every history's list)
+            const BrickGeom bg{(uint32_t)P.columns, (uint32_t)P.rows, (uint32_t)D.nbx, (uint32_t)D.nby};
+            int64_t prev = 0;
+            for (int64_t gi = 0; gi <= ng; gi++) {
+                const int64_t a = prev, b = gi < ng ? D.groups[gi].hist_begin : n;
+                if (b > a)
+                    hipLaunchKernelGGL(k_bricks_list, dim3(pct_grid_capped((b - a) * 16, kBlock, 1u << 20)), dim3(kBlock),
+                                       0, s, b - a, ctx->list_off + a, ctx->list_cnt + a, ctx->list_vox, D.vox_b, bg,
+                                       (uint32_t)P.num_voxels);
+                if (gi < ng) prev = D.groups[gi].hist_end > prev ? D.groups[gi].hist_end : prev;
+            }
+            PCT_CHECK_HIP(ctx, hipGetLastError());
+        }
+        PCT_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream2));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, q.t0, q.t1);
+        ctx->t_mlp_kernel = ms;  // first MLP launch to the last one's end, sharing the GPU with the transposition
+        if (bricks) D.vox_b_ready = true;
+    }
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
     pct_free(&proc_buf.p);
+    pct_free(&e_buf.p);
     // per block: its runs [run_start, next run_start) within the group
     for (auto& bk : D.blocks) {
         const DropPlan::Group& g = D.groups[bk.group];

@@ -707,30 +707,56 @@ int pct_stage_build_lists(pct_ctx* ctx)
         ctx->list_cap = total;
     }
     ctx->plan.vox_b_ready = false;  // the brick copy of the lists belongs to the previous build
-    if (n > 0) {
-        size_t lds = (size_t)a.n_steps * 6 * sizeof(double);
-        if (lds > 64 * 1024) {
-            ctx->err = "MLP: too many u-steps for the LDS step table";
-            return PCT_E_LIMIT;
+    const size_t lds = (size_t)a.n_steps * 6 * sizeof(double);
+    if (lds > 64 * 1024) {
+        ctx->err = "MLP: too many u-steps for the LDS step table";
+        return PCT_E_LIMIT;
+    }
+    int dev_cus = 256;
+    (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    // the MLP list kernel over the histories [h0, h1): per-history arrays offset by h0, list offsets absolute
+    const MlpLaunch launch = [&, lds, dev_cus](int64_t h0, int64_t h1, hipStream_t st) -> int {
+        const int64_t m = h1 - h0;
+        if (m <= 0) return PCT_OK;
+        H10 hq;
+        for (int k = 0; k < 10; k++) hq.h[k] = ctx->h[k] + h0;
+        const int64_t want = (m + kBlock - 1) / kBlock;
+        // persistent grid (8 workgroups per CU) when the kernel has the GPU to itself; one lane per history
+        // when it shares the GPU with the transposition, so that the two streams' workgroups interleave
+        const int per_cu = ctx->mlp_blocks_per_cu >= 0 ? ctx->mlp_blocks_per_cu : (st == ctx->stream ? 8 : 0);
+        const int64_t cap_blocks = per_cu > 0 ? (int64_t)dev_cus * per_cu : want;
+        const unsigned blocks = (unsigned)(want < cap_blocks ? want : cap_blocks);
+        if (standard)
+            hipLaunchKernelGGL(k_mlp_lists_std, dim3(blocks), dim3(kBlock), 0, st, m, sa, hq, ctx->first_voxel + h0,
+                               ctx->list_off + h0, ctx->list_cnt + h0, ctx->list_vox, ctx->chord + h0,
+                               ctx->list_overflow);
+        else
+            hipLaunchKernelGGL(k_mlp_lists, dim3(blocks), dim3(kBlock), lds, st, m, a, hq, ctx->first_voxel + h0,
+                               ctx->T.sin_t, ctx->T.cos_t, ctx->T.scat, ctx->T.p12, ctx->T.p26, ctx->T.p312,
+                               ctx->T.step_terms, ctx->list_off + h0, ctx->list_cnt + h0, ctx->list_vox,
+                               ctx->chord + h0, ctx->list_overflow);
+        PCT_CHECK_HIP(ctx, hipGetLastError());
+        return PCT_OK;
+    };
+    // Resident lists with the transposed DROP plan: the plan build runs the MLP kernel group by group on a
+    // second stream, overlapping the transposition of the groups before (no segments, no bucket
+    // transposition: those build their plan after the lists).
+    const bool pipe = ctx->drop_pipeline && !ctx->seg_inside && !ctx->seg_lists_only &&
+                      !(ctx->drop_flags & PCT_DROP_BUCKETS) && n > 0;
+    if (pipe) {
+        ctx->stage = ST_LISTS;
+        ctx->plan.ready = false;
+        const int rc = pct_drop_prepare(ctx, &launch);
+        if (rc != PCT_OK) {
+            ctx->stage = ST_ENDPOINTS;  // the lists are incomplete
+            return rc;
         }
-        int dev_cus = 256;
-        (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-        int64_t want = (n + kBlock - 1) / kBlock;
-        int64_t cap_blocks = (int64_t)dev_cus * 8;
-        unsigned blocks = (unsigned)(want < cap_blocks ? want : cap_blocks);
+    } else if (n > 0) {
         hipEvent_t e0, e1;
         PCT_CHECK_HIP(ctx, hipEventCreate(&e0));
         PCT_CHECK_HIP(ctx, hipEventCreate(&e1));
         PCT_CHECK_HIP(ctx, hipEventRecord(e0, s));
-        if (standard)
-            hipLaunchKernelGGL(k_mlp_lists_std, dim3(blocks), dim3(kBlock), 0, s, n, sa, hp, ctx->first_voxel,
-                               ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord, ctx->list_overflow);
-        else
-            hipLaunchKernelGGL(k_mlp_lists, dim3(blocks), dim3(kBlock), lds, s, n, a, hp, ctx->first_voxel,
-                               ctx->T.sin_t, ctx->T.cos_t, ctx->T.scat, ctx->T.p12, ctx->T.p26, ctx->T.p312,
-                               ctx->T.step_terms, ctx->list_off, ctx->list_cnt, ctx->list_vox, ctx->chord,
-                               ctx->list_overflow);
-        PCT_CHECK_HIP(ctx, hipGetLastError());
+        PCT_TRY(launch(0, n, s));
         PCT_CHECK_HIP(ctx, hipEventRecord(e1, s));
         PCT_CHECK_HIP(ctx, hipEventSynchronize(e1));
         float ms = 0;
@@ -744,6 +770,8 @@ int pct_stage_build_lists(pct_ctx* ctx)
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
     if (overflow) {
         ctx->err = "MLP list capacity exceeded (code " + std::to_string(overflow) + ")";
+        ctx->plan.ready = false;  // a plan built with the lists holds them
+        ctx->stage = ST_ENDPOINTS;
         return PCT_E_LIMIT;
     }
     // entries actually listed (the capacity `total` counts every u-step)
@@ -758,6 +786,7 @@ int pct_stage_build_lists(pct_ctx* ctx)
     ctx->list_entries = (int64_t)listed;
     ctx->list_slots = total;
     ctx->stage = ST_LISTS;
+    if (pipe) return PCT_OK;  // the plan was built with the lists
     // the DROP blocks' transposed lists belong to the list build
     ctx->plan.ready = false;
     if (!ctx->seg_lists_only) PCT_TRY(pct_drop_prepare(ctx));
