@@ -19,9 +19,25 @@ namespace {
 constexpr int kBlock = 256;
 inline unsigned grid_for(int64_t n, int block = kBlock) { return pct_grid_for(n, block); }
 
+// The hull as one bit per voxel (bit v % 32 of word v / 32 set where hull[v] == 1): 1/8 of the bytes, so
+// the whole mask stays in each XCD's L2 while the endpoint walks test it.
+__global__ void k_hull_bits(int64_t nv, const uint8_t* __restrict__ hull, uint32_t* __restrict__ bits)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w * 32 >= nv) return;
+    uint32_t b = 0;
+    for (int k = 0; k < 32 && w * 32 + k < nv; k++) b |= (uint32_t)(hull[w * 32 + k] == 1) << k;
+    bits[w] = b;
+}
+
+__device__ __forceinline__ int hull_bit(const uint32_t* __restrict__ bits, int64_t v)
+{
+    return (int)((bits[v >> 5] >> (v & 31)) & 1u);
+}
+
 // find_MLP_endpoints_GPU, DS.cu:7581-7773; the walk is bounded by 4*(C+R+S)+16 steps (the
 // reference loops forever when the walk never leaves the image, e.g. z clamped at slice 0).
-__device__ int find_endpoint(const DevGeom& g, const uint8_t* __restrict__ hull, double x_start, double y_start,
+__device__ int find_endpoint(const DevGeom& g, const uint32_t* __restrict__ hull, double x_start, double y_start,
                              double z_start, double xy_angle, double xz_angle, double* xo, double* yo, double* zo,
                              int* vxo, int* vyo, int* vzo, int entering)
 {
@@ -50,7 +66,7 @@ __device__ int find_endpoint(const DevGeom& g, const uint8_t* __restrict__ hull,
     double dy_dz = (double)pct_tanf(xy_angle) / txz;
     int hit = 0;
     bool outside = (vx >= C) || (vy >= R) || (vz >= S) || (vx < 0) || (vy < 0) || (vz < 0);
-    if (!outside) hit = (hull[voxel] == 1);
+    if (!outside) hit = hull_bit(hull, voxel);
     bool end_walk = outside || hit;
     const int cap = 4 * (C + R + S) + 16;
     int steps = 0;
@@ -107,7 +123,7 @@ __device__ int find_endpoint(const DevGeom& g, const uint8_t* __restrict__ hull,
         }
         voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
         outside = (vx >= C) || (vy >= R) || (vz >= S) || (vx < 0) || (vy < 0) || (vz < 0);
-        if (!outside) hit = (hull[voxel] == 1);
+        if (!outside) hit = hull_bit(hull, voxel);
         end_walk = outside || hit;
         if (++steps > cap) {
             hit = 0;
@@ -132,7 +148,7 @@ struct H10 {
 // collect_MLP_endpoints_GPU_nobool, DS.cu:7819-7854.  With `perm`, lane t takes history perm[t]: in
 // sinogram-bin order the lanes of a wavefront walk nearly the same chords, so their walks to the hull take
 // about as many steps.
-__global__ void k_endpoints(int64_t n, const uint32_t* __restrict__ perm, DevGeom g, const uint8_t* __restrict__ hull,
+__global__ void k_endpoints(int64_t n, const uint32_t* __restrict__ perm, DevGeom g, const uint32_t* __restrict__ hull,
                             H10 hp, uint32_t* first_voxel, uint8_t* keep)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -596,10 +612,15 @@ int pct_stage_endpoints(pct_ctx* ctx)
         PCT_TRY(pct_bin_order(ctx, n));
         perm = ctx->u32_d;
     }
-    if (n > 0)
-        hipLaunchKernelGGL(k_endpoints, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, g,
-                           ctx->hull_endpoints ? ctx->hull_endpoints : ctx->hull, hp, ctx->first_voxel,
-                           ctx->flag);
+    pct_tmp<uint32_t> bits;
+    if (n > 0) {
+        const int64_t NV = P.num_voxels, words = (NV + 31) / 32;
+        PCT_TRY(pct_alloc(ctx, &bits.p, words));
+        hipLaunchKernelGGL(k_hull_bits, dim3(grid_for(words)), dim3(kBlock), 0, s, NV,
+                           ctx->hull_endpoints ? ctx->hull_endpoints : ctx->hull, bits.p);
+        hipLaunchKernelGGL(k_endpoints, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, g, bits.p, hp,
+                           ctx->first_voxel, ctx->flag);
+    }
     PCT_CHECK_HIP(ctx, hipGetLastError());
     int64_t kept = 0;
     PCT_TRY(pct_compact(ctx, ctx->flag, n, &kept, true));  // DS.cu:8286-8294
