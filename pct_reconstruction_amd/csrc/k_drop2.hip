@@ -544,9 +544,9 @@ __device__ __forceinline__ void term_bounds(float f, int& top, int& lsb)
 // list in order (DS.cu:9245), and the regrouped sum equals the ordered one whenever no partial sum rounds:
 // with every term a multiple of 2^lsb and below 2^top in magnitude, any partial sum of at most cnt terms
 // is a multiple of 2^lsb below 2^(top + ceil(log2 cnt)), exact in double when top + ceil(log2 cnt) - lsb
-// <= 53.  Rows that fail the test (none on the head-50M images) are summed again in order by lane 0.
-// The sum is valid in lane 0.  (Tracking the bounds this way costs about 1 us per DROP block; comparing
-// raw bit patterns instead measured 7 us.)
+// <= 53.  The G lanes of a failing row gather its list again in chunks and all add the chunk in list order
+// from the lanes' registers.  The sum is valid in lane 0.  (Tracking the bounds this way costs about 1 us
+// per DROP block; comparing raw bit patterns instead measured 7 us.)
 template <int G>
 __device__ inline double row_sum(const uint32_t* __restrict__ lst, int cnt, const float* __restrict__ x, int lane)
 {
@@ -574,9 +574,24 @@ __device__ inline double row_sum(const uint32_t* __restrict__ lst, int cnt, cons
         lsb = l < lsb ? l : lsb;
     }
     const int lg = cnt > 1 ? 32 - __clz((uint32_t)cnt - 1u) : 0;  // ceil(log2 cnt)
-    if (lane == 0 && top + lg - lsb > 53) {                          // all-zero rows (top -1000) never fail
+    if (top + lg - lsb > 53) {  // the same in the G lanes after the butterfly; all-zero rows (top -1000) never fail
+        // the ordered sum: chunks of kUnroll * G entries gathered as above (entry j + q G + l in lane l), then
+        // added in list order by every lane from the lanes' registers
         double s = 0.0;
-        for (int j = 0; j < cnt; j++) s += (double)x[lst[j]];
+        for (int j = 0; j < cnt; j += kUnroll * G) {
+            float xv[kUnroll];
+#pragma unroll
+            for (int q = 0; q < kUnroll; q++) {
+                const int e = j + q * G + lane;
+                xv[q] = e < cnt ? x[lst[e]] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < kUnroll; q++)
+                for (int l = 0; l < G; l++) {
+                    const float v = __shfl(xv[q], l, G);
+                    if (j + q * G + l < cnt) s += (double)v;
+                }
+        }
         part = s;
     }
     return part;
