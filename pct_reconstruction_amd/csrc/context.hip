@@ -127,6 +127,7 @@ int pct_ctx_create(const pct_params* p, int device, pct_ctx** out)
         if (t == 256 || t == 512) ctx->carve_block = t;
     }
     if (const char* e = std::getenv("PCT_ENDPOINT_ORDER")) ctx->endpoint_order = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PCT_CARVE_LINES")) ctx->carve_line_order = std::atoi(e) != 0;
     if (const char* e = std::getenv("PCT_CARVE_SPILL")) {
         int t = std::atoi(e);
         if (t >= 1 && t <= 512) ctx->carve_spill = t;

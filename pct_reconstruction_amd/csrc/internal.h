@@ -175,6 +175,7 @@ struct pct_ctx {
     int64_t bin_perm_n = -1;  // u32_d holds the bin order of the first bin_perm_n histories (pct_bin_order)
     int carve_spill = 512;    // LDS count at which a carve visit goes to global memory, 1..512 (PCT_CARVE_SPILL)
     int carve_table = 8192;   // LDS hash slots of the fused carve (4096, 8192, 16384); 0: one walk per method (PCT_CARVE_TABLE)
+    int carve_line_order = 1;  // hull carving in chord-line order (both directions of a line together; PCT_CARVE_LINES)
     int endpoint_order = 1;   // MLP endpoints in sinogram-bin order (PCT_ENDPOINT_ORDER)
     int carve_block = 512;    // lanes per workgroup of the fused carve (256, 512; PCT_CARVE_BLOCK)
 
@@ -329,6 +330,7 @@ int pct_compact(pct_ctx* ctx, const uint8_t* keep, int64_t n, int64_t* n_out, bo
 // per-history arrays (and their compaction scratch) for n histories
 int pct_ensure_hist_capacity(pct_ctx* ctx, int64_t n);
 int pct_bin_order(pct_ctx* ctx, int64_t n);
+int pct_line_order(pct_ctx* ctx, int64_t n);
 // per-kernel timing of DROP launches (HIP events around every launch on the launch stream)
 struct DropTiming {
     double scatter_ms = 0, apply_ms = 0;

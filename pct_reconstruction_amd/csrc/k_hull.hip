@@ -377,7 +377,7 @@ int pct_carve_counts(pct_ctx* ctx)
         if (ctx->carve_table && NV < (int64_t)kEmpty && n <= 0x7fffffff) {
             // histories in sinogram-bin order (a sort of up to 2^31 - 1); voxel numbers are 32-bit table keys
             // below kEmpty
-            PCT_TRY(pct_bin_order(ctx, n));
+            PCT_TRY(ctx->carve_line_order ? pct_line_order(ctx, n) : pct_bin_order(ctx, n));
             Gates gt{P.sc_on, P.msc_on, P.sm_on, P.sc_lower_threshold, P.sc_upper_threshold, P.msc_lower_threshold,
                      P.msc_upper_threshold, P.sm_lower_threshold, (uint32_t)ctx->carve_spill};
             const int tb = ctx->carve_table, bs = ctx->carve_block;

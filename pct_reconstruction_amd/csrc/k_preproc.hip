@@ -375,6 +375,46 @@ int pct_bin_order(pct_ctx* ctx, int64_t n)
     return PCT_OK;
 }
 
+// Sort key of a history's chord: the bin with angle bins a and a + A/2 paired (the same line traversed in
+// the opposite direction, at t mirrored), so that histories on one line, either way, become neighbours.
+__global__ void k_line_keys(int64_t n, const int32_t* __restrict__ bin_num, int T, int A, uint32_t* __restrict__ key)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = (uint32_t)bin_num[i];
+    const uint32_t t = b % (uint32_t)T, r = b / (uint32_t)T, a = r % (uint32_t)A, v = r / (uint32_t)A;
+    const uint32_t h = (uint32_t)A / 2u, half = a >= h ? 1u : 0u;
+    const uint32_t tc = half ? (uint32_t)T - 1u - (t < (uint32_t)T ? t : (uint32_t)T - 1u) : t;
+    key[i] = (((v * h + (a - half * h)) * (uint32_t)T + tc) << 1) | half;
+}
+
+// History indices ordered by chord line (k_line_keys) into ctx->u32_d.  Replaces a bin order held there.
+int pct_line_order(pct_ctx* ctx, int64_t n)
+{
+    const pct_params& P = ctx->P;
+    hipStream_t s = ctx->stream;
+    if (n > 0x7fffffff || (P.angular_bins & 1)) return pct_bin_order(ctx, n);
+    ctx->bin_perm_n = -1;
+    int end_bit = 2;
+    while ((1ll << end_bit) < 2ll * P.num_bins) end_bit++;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_line_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, ctx->bin_num, P.t_bins,
+                           P.angular_bins, ctx->u32_a);
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(kBlock), 0, s, n, ctx->u32_c);
+        size_t need = 0;
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, need, ctx->u32_a, ctx->u32_b, ctx->u32_c,
+                                                              ctx->u32_d, (int)n, 0, end_bit, s));
+        if (need > ctx->cub_tmp_bytes) {
+            PCT_TRY(pct_alloc(ctx, (uint8_t**)&ctx->cub_tmp, need));
+            ctx->cub_tmp_bytes = need;
+        }
+        PCT_CHECK_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, need, ctx->u32_a, ctx->u32_b, ctx->u32_c,
+                                                              ctx->u32_d, (int)n, 0, end_bit, s));
+        PCT_CHECK_HIP(ctx, hipGetLastError());
+    }
+    return PCT_OK;
+}
+
 int pct_ensure_hist_capacity(pct_ctx* ctx, int64_t n)
 {
     if (n <= ctx->cap_hist) return PCT_OK;
