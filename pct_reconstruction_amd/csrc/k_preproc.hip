@@ -216,14 +216,33 @@ __device__ __forceinline__ double wrap_pi(double a, double PI)
     return a;
 }
 
+// The per-history inputs of the bin statistics in bin order (j = position in the sorted order): in-range
+// flag, WEPL and the two wrapped angle differences, so that the three passes of k_bin_stats stream
+// contiguous memory instead of gathering through the order.
+__global__ void k_bin_gather(int64_t n, const uint32_t* __restrict__ perm, const uint8_t* __restrict__ in_range,
+                             const float* __restrict__ wepl, const float* __restrict__ xya_e,
+                             const float* __restrict__ xza_e, const float* __restrict__ xya_x,
+                             const float* __restrict__ xza_x, double PI, uint8_t* __restrict__ g_ir,
+                             float* __restrict__ g_w, double* __restrict__ g_ut, double* __restrict__ g_uv)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t i = perm[j];
+    g_ir[j] = in_range[i];
+    g_w[j] = wepl[i];
+    g_ut[j] = wrap_pi((double)(xya_x[i] - xya_e[i]), PI);
+    g_uv[j] = wrap_pi((double)(xza_x[i] - xza_e[i]), PI);
+}
+
 // One lane per bin: binning sums (DS.cu:5518-5546), calculate_means_GPU (:5644), the squared
 // deviations (:5689-5717), calculate_standard_deviations_GPU (:5737) and statistical_cuts_GPU
-// (:5937-5971), each over the bin's histories in ascending history order.
-__global__ void k_bin_stats(int NB, const int32_t* off, const uint32_t* perm, const uint8_t* in_range,
-                            const float* wepl, const float* xya_e, const float* xza_e, const float* xya_x,
-                            const float* xza_x, int32_t* counts_pre, float* m_w, float* m_ut, float* m_uv,
-                            float* s_w, float* s_ut, float* s_uv, int32_t* counts_post, float* sinogram,
-                            uint8_t* failed, double PI, float k, int quirk_a1)
+// (:5937-5971), each over the bin's histories in ascending history order, read from the bin-ordered
+// inputs of k_bin_gather.
+__global__ void k_bin_stats(int NB, const int32_t* off, const uint32_t* perm, const uint8_t* __restrict__ g_ir,
+                            const float* __restrict__ g_w, const double* __restrict__ g_ut,
+                            const double* __restrict__ g_uv, int32_t* counts_pre, float* m_w, float* m_ut,
+                            float* m_uv, float* s_w, float* s_ut, float* s_uv, int32_t* counts_post, float* sinogram,
+                            uint8_t* failed, float k, int quirk_a1)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= NB) return;
@@ -231,14 +250,11 @@ __global__ void k_bin_stats(int NB, const int32_t* off, const uint32_t* perm, co
     int cnt = 0;
     float sw = 0.0f, su = 0.0f, sv = 0.0f;
     for (int32_t j = beg; j < end; j++) {
-        uint32_t i = perm[j];
-        if (!in_range[i]) continue;
-        double rel_ut = wrap_pi((double)(xya_x[i] - xya_e[i]), PI);
-        double rel_uv = wrap_pi((double)(xza_x[i] - xza_e[i]), PI);
+        if (!g_ir[j]) continue;
         cnt += 1;
-        sw += wepl[i];
-        su += (float)rel_ut;
-        sv += (float)rel_uv;
+        sw += g_w[j];
+        su += (float)g_ut[j];
+        sv += (float)g_uv[j];
     }
     if (cnt > 0) {
         sw /= cnt;
@@ -251,13 +267,10 @@ __global__ void k_bin_stats(int NB, const int32_t* off, const uint32_t* perm, co
     m_uv[b] = sv;
     float qw = 0.0f, qu = 0.0f, qv = 0.0f;
     for (int32_t j = beg; j < end; j++) {
-        uint32_t i = perm[j];
-        if (!quirk_a1 && !in_range[i]) continue;
-        double rel_ut = wrap_pi((double)(xya_x[i] - xya_e[i]), PI);
-        double rel_uv = wrap_pi((double)(xza_x[i] - xza_e[i]), PI);
-        double d_w = (double)(wepl[i] - sw);
-        double d_ut = rel_ut - su;
-        double d_uv = rel_uv - sv;
+        if (!quirk_a1 && !g_ir[j]) continue;
+        double d_w = (double)(g_w[j] - sw);
+        double d_ut = g_ut[j] - su;
+        double d_uv = g_uv[j] - sv;
         qw += pct_sqf(d_w);
         qu += pct_sqf(d_ut);
         qv += pct_sqf(d_uv);
@@ -273,21 +286,20 @@ __global__ void k_bin_stats(int NB, const int32_t* off, const uint32_t* perm, co
     int cpost = 0;
     float sino = 0.0f;
     for (int32_t j = beg; j < end; j++) {
-        uint32_t i = perm[j];
-        if (!quirk_a1 && !in_range[i]) {
+        const uint32_t i = perm[j];
+        if (!quirk_a1 && !g_ir[j]) {
             failed[i] = 1;
             continue;
         }
-        double rel_ut = wrap_pi((double)(xya_x[i] - xya_e[i]), PI);
-        double rel_uv = wrap_pi((double)(xza_x[i] - xza_e[i]), PI);
-        int pass_ut = fabs(rel_ut - su) < (double)(k * qu);
-        int pass_uv = fabs(rel_uv - sv) < (double)(k * qv);
-        int pass_w = fabsf(sw - wepl[i]) <= k * qw;
+        const float w = g_w[j];
+        int pass_ut = fabs(g_ut[j] - su) < (double)(k * qu);
+        int pass_uv = fabs(g_uv[j] - sv) < (double)(k * qv);
+        int pass_w = fabsf(sw - w) <= k * qw;
         int f = !pass_ut || !pass_uv || !pass_w;
         failed[i] = (uint8_t)f;
         if (!f) {
             cpost += 1;
-            sino += wepl[i];
+            sino += w;
         }
     }
     counts_post[b] = cpost;
@@ -531,10 +543,20 @@ int pct_stage_stats_and_cuts(pct_ctx* ctx)
     PCT_CHECK_HIP(ctx, hipStreamSynchronize(s));
     uint8_t* failed = ctx->flag;
     if (n > 0) PCT_CHECK_HIP(ctx, hipMemsetAsync(failed, 0, n, s));
-    hipLaunchKernelGGL(k_bin_stats, dim3(grid_for(NB)), dim3(kBlock), 0, s, NB, ctx->bin_off, perm, ctx->in_range,
-                       ctx->wepl, ctx->h[6], ctx->h[7], ctx->h[8], ctx->h[9], ctx->counts_pre, ctx->mean[0],
-                       ctx->mean[1], ctx->mean[2], ctx->stddev[0], ctx->stddev[1], ctx->stddev[2], ctx->counts_post,
-                       ctx->sinogram, failed, P.pi, (float)P.sigmas_2_keep, P.quirk_a1_bin0);
+    pct_tmp<uint8_t> g_ir;
+    pct_tmp<float> g_w;
+    pct_tmp<double> g_ut, g_uv;
+    PCT_TRY(pct_alloc(ctx, &g_ir.p, n > 0 ? n : 1));
+    PCT_TRY(pct_alloc(ctx, &g_w.p, n > 0 ? n : 1));
+    PCT_TRY(pct_alloc(ctx, &g_ut.p, n > 0 ? n : 1));
+    PCT_TRY(pct_alloc(ctx, &g_uv.p, n > 0 ? n : 1));
+    if (n > 0)
+        hipLaunchKernelGGL(k_bin_gather, dim3(g), dim3(kBlock), 0, s, n, perm, ctx->in_range, ctx->wepl, ctx->h[6],
+                           ctx->h[7], ctx->h[8], ctx->h[9], P.pi, g_ir.p, g_w.p, g_ut.p, g_uv.p);
+    hipLaunchKernelGGL(k_bin_stats, dim3(grid_for(NB)), dim3(kBlock), 0, s, NB, ctx->bin_off, perm, g_ir.p, g_w.p,
+                       g_ut.p, g_uv.p, ctx->counts_pre, ctx->mean[0], ctx->mean[1], ctx->mean[2], ctx->stddev[0],
+                       ctx->stddev[1], ctx->stddev[2], ctx->counts_post, ctx->sinogram, failed,
+                       (float)P.sigmas_2_keep, P.quirk_a1_bin0);
     PCT_CHECK_HIP(ctx, hipGetLastError());
     // keep the histories that passed (DS.cu:5892-5911)
     if (n > 0) hipLaunchKernelGGL(k_not, dim3(g), dim3(kBlock), 0, s, n, failed, ctx->u8_tmp);
