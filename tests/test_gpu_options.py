@@ -118,8 +118,9 @@ def test_carve_variants_count_identically(oracle, monkeypatch, table, block, spi
         assert r.count("post_cuts") == ref["h_cut"].shape[1]
 
 
-@pytest.mark.parametrize("group_entries", ["1073741824", "150000"])
-def test_pipelined_list_build_matches_the_serial_one(monkeypatch, group_entries):
+@pytest.mark.parametrize("group_entries,launch", [("1073741824", {}), ("150000", {}),
+                                                  ("150000", {"bricks": False, "spatial_order": False})])
+def test_pipelined_list_build_matches_the_serial_one(monkeypatch, group_entries, launch):
     """The MLP lists built group by group on a second stream while earlier groups are transposed give the
     same lists, the same DROP image and the same residual as the lists built before the plan.  Blocks of
     3200 histories at 700 per launch leave unprocessed tails in every block and a last block of 527
@@ -132,6 +133,8 @@ def test_pipelined_list_build_matches_the_serial_one(monkeypatch, group_entries)
     for pipe in ("0", "1"):
         monkeypatch.setenv("PCT_DROP_PIPELINE", pipe)
         with Reconstruction(P) as r:
+            if launch:
+                r.set_launch(**launch)
             r.load_histories(data["t"], data["v"], data["u"], data["wepl"], data["angle_deg"])
             r.preprocessing()
             r.reconstruction_cuts()
@@ -141,7 +144,7 @@ def test_pipelined_list_build_matches_the_serial_one(monkeypatch, group_entries)
             out[pipe] = dict(off=off, vox=vox, chord=r.histories("chord"), x=r.image("x").reshape(-1),
                              res=r.residual(), entries=r.count("list_entries"), n=r.count("reconstruction"))
     a, b = out["0"], out["1"]
-    print(f"[pipelined lists, group entries {group_entries}] {a['n']} histories, {a['entries']} entries, "
+    print(f"[pipelined lists, group entries {group_entries}, launch {launch}] {a['n']} histories, {a['entries']} entries, "
           f"last block {a['n'] % 3200} histories", flush=True)
     assert a["n"] % 3200 < 700  # the last block's histories are not processed (quirk A3) and join no group
     for k in ("off", "vox", "chord", "x"):
