@@ -10,12 +10,13 @@
 // radix sort by (block, voxel) is stable, so equal keys keep history order; blocks are sorted in
 // groups of up to 2^30 entries).  Then each iteration runs, per block,
 //   k_drop_rows   G lanes per history: coalesced reads of its voxel list, gather of x, FP64
-//                 reduction of a_i.x, u_i stored in r[rank];
+//                 reduction of a_i.x (regrouped over the lanes only where no partial sum can round,
+//                 else in list order), u_i stored in r[rank];
 //   k_drop_cols   one lane per touched voxel: sum of r over its histories in history order
 //                 (FP32 as the reference, FP32 with Kahan compensation, or FP64), S = the number of
 //                 entries, x[v] += sum / S.
-// No atomics, no scratch clearing, and the per-voxel sums are formed in the same order as a
-// single-threaded run of the reference: the result is reproducible bit for bit.  By default the
+// No atomics, no scratch clearing, and the sums equal those of a single-threaded run of the reference:
+// the result is reproducible bit for bit.  By default the
 // column kernel reads the ranks from a sliced copy of the transposed lists (column-major per 64
 // touched voxels, the voxels of one wavefront), so that each rank load is one coalesced read.
 //

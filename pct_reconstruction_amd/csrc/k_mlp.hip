@@ -1,15 +1,17 @@
 // k_mlp.hip — stage (d): MLP endpoints on the hull and the MLP voxel lists.
 //
 // Endpoints: one lane per history, two straight walks along the tracker angles to the first hull
-// voxel (find_MLP_endpoints_GPU, DS.cu:7581-7773).
+// voxel (find_MLP_endpoints_GPU, DS.cu:7581-7773), the histories in sinogram-bin order and the hull
+// as a bit mask.
 //
-// MLP lists: a persistent one-history-per-lane kernel.  The Sigma_1 terms of every u-step (they
-// depend only on the step number) are precomputed on the host in FP64 and staged in LDS at kernel
-// start; per step a lane reads them from LDS, one scattering coefficient for its Sigma_2 from the
-// FP64 table in HBM/L2, and does the 2x2 inversions and the t/v solve in FP64 exactly as
-// find_MLP_path_GPU_tabulated (DS.cu:9104-9262).  Each history's voxel list is written to its own
+// MLP lists: a one-history-per-lane kernel.  The Sigma_1 terms of every u-step (they depend only on
+// the step number) are precomputed on the host in FP64 and staged in LDS at kernel start; per step a
+// lane reads them from LDS, one scattering coefficient for its Sigma_2 from the FP64 table in HBM/L2,
+// and does the 2x2 inversions and the t/v solve in FP64 exactly as find_MLP_path_GPU_tabulated
+// (DS.cu:9104-9262).  Each history's voxel list is written, in 64-byte chunks staged in LDS, to its own
 // segment of a CSR array whose capacity comes from the number of u-steps; the DROP kernels read
-// entries [off, off + count).
+// entries [off, off + count).  With the transposed DROP plan the kernel runs group by group on a second
+// stream while the plan's earlier groups are transposed (pct_drop_prepare).
 #include <hipcub/hipcub.hpp>
 
 #include "internal.h"
